@@ -1,0 +1,158 @@
+#!/usr/bin/env python
+"""Construction benchmark: RecSplit MPHF of n = 5,000,000 keys, l = 16, b = 2000.
+
+One step = one complete GPURecSplit build (bucket partition, every seed search,
+Golomb-Rice + Elias-Fano encoding, copy of the structure to the host) of the
+5M splitmix64(seed 1) keys, which are resident in HBM before timing starts.
+Prints one JSON line (metric ns/object, lower is better) with:
+  roofline      dominant search kernel against the int32 VALU issue peak
+  cpu_baseline  the CPU oracle (SimdRecSplit algorithm, vectorised C, all
+                threads used) on a bounded sample of the same workload
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# MI355X int32 VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12
+# full-rate lane operations per second (the published 157.3 TFLOP/s FP32 vector
+# figure counts packed FMA: x2 flops, x2 packed).
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+# VALU issue cycles per key-seed evaluation of each search kernel's inner loop,
+# counted from the gfx950 assembly of librecsplit_amd.so (full-rate op = 1,
+# v_mul_lo_u32 = 4).  Index: fanout-2, upper aggr, lower aggr, rotation leaf, brute leaf.
+OPS_PER_EVAL = {"fanout2": 17, "upper_aggr": 20, "lower_aggr": 20, "leaf_rotation": 22, "leaf_bruteforce": 19}
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64_keys(seed: int, n: int) -> np.ndarray:
+    """(n, 2) uint64 keys: first = next(), second = next() of SplitMix64(seed)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + GAMMA * np.arange(1, 2 * n + 1, dtype=np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z.reshape(n, 2)
+
+
+def cpu_baseline(leaf: int, bucket: int, seed: int) -> dict:
+    """Oracle (vectorised C restatement of SimdRecSplit) on a bounded sample:
+    5 buckets' worth of keys per thread of the same generator."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import OracleMPHF, splitmix_keys
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    n_s = bucket * 5 * threads
+    keys = splitmix_keys(seed, n_s)
+    t0 = time.perf_counter()
+    rs = OracleMPHF.build(keys, leaf, bucket, rotate=True, threads=threads)
+    dt = time.perf_counter() - t0
+    assert rs.bijection_errors(keys) == 0
+    return {"value": dt / n_s * 1e9, "unit": "ns/object", "cores": threads, "kind": "port",
+            "sample": f"first {n_s} keys of splitmix64(seed {seed}) ({n_s // bucket} buckets of {bucket}), "
+                      f"l={leaf}, rotation on, {threads} threads, wall {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=5_000_000)
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--bucket", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(buckets shard by contiguous ranges across GPUs; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    import gpurecsplit_amd as gra
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    keys = splitmix64_keys(args.seed, args.n)
+    d_keys = torch.from_numpy(keys.view(np.int64)).to("cuda:0")
+    ctx = gra.Context(0)
+    print(f"# keys resident: n={args.n} l={args.leaf} b={args.bucket}", file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        rs = gra.GPURecSplit(d_keys, args.bucket, leaf_size=args.leaf, ctx=ctx)
+        print(f"# warmup {i}: {rs.stats()['total_ms']:.1f} ms", file=sys.stderr, flush=True)
+        del rs
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        rs = gra.GPURecSplit(d_keys, args.bucket, leaf_size=args.leaf, ctx=ctx)
+        stats.append(rs.stats())
+        print(f"# step {i}: {stats[-1]['total_ms']:.1f} ms", file=sys.stderr, flush=True)
+        if i + 1 < args.steps:
+            del rs
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # verification outside the timed region: bijection through the GPU query kernel
+    idx = rs.query_device(d_keys)
+    ok = bool(torch.equal(torch.sort(idx).values, torch.arange(args.n, device=idx.device)))
+    bits_per_key = rs.get_bits() / args.n
+
+    # dominant search kernel: the kind with the most event-timed search time
+    last = stats[-1]
+    kind = max(last["search_ms"], key=lambda k: last["search_ms"][k])
+    kms = np.mean([s["search_ms"][kind] for s in stats])
+    evals = last["evals"][kind]
+    achieved = evals * OPS_PER_EVAL[kind] / (kms / 1e3) / 1e12
+    total_evals = sum(last["evals"].values())
+    search_total_ms = sum(last["search_ms"].values())
+
+    line = {
+        "metric": "construction ns/object at l=16,b=2000,n=5M (batched vs plain); p99 job latency",
+        "value": ms_per_step * 1e6 / args.n,
+        "unit": "ns/object",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64",
+        "data": "synthetic (splitmix64 128-bit key hashes)",
+        "config": {"workload": f"plain GPURecSplit<{args.leaf}> build, n={args.n}, b={args.bucket}, "
+                               f"rotation on, splitmix64 seed {args.seed}", "mode": "plain",
+                   "leaf": args.leaf, "bucket": args.bucket, "n": args.n},
+        "roofline": {"bound": "valu", "kernel": f"k_search<{kind}>", "achieved": achieved,
+                     "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
+                     "traffic": None, "evals": evals, "kernel_ms": kms,
+                     "ops_per_eval": OPS_PER_EVAL[kind]},
+        "seeds_evals_per_s": total_evals / (search_total_ms / 1e3),
+        "bits_per_key": bits_per_key,
+        "bijection_verified": ok,
+        "phase_ms": {"partition": last["partition_ms"], "search": last["search_ms"], "split": last["split_ms"],
+                     "encode": last["encode_ms"], "total": last["total_ms"]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.leaf, args.bucket, args.seed)
+    print(json.dumps(line))
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

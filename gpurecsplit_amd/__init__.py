@@ -1,0 +1,237 @@
+"""MI355X-native RecSplit minimal perfect hash construction.
+
+Python mirror of the reference's GPURecSplit surface over the C ABI of
+librecsplit_amd.so (include/gpurecsplit_amd.h):
+
+    rs = GPURecSplit(keys, bucket_size, leaf_size=16)   # GPURecSplit<16>(keys, b, t)
+    rs(key) / rs.query(keys)                            # operator()(const hash128_t&)
+    rs.serialize() / GPURecSplit.deserialize(...)       # operator<< / operator>>
+    rs.get_bits()                                       # getBits()
+
+Keys are 128-bit hashes given as an (n, 2) uint64 array of (first, second)
+pairs, the memory layout of the reference's hash128_t (include/util/Hash128.h:14-23).
+The GPU build is the only construction path: there is no CPU fallback, and the
+import fails loudly when the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["GPURecSplit", "RecSplitError", "Context", "golomb_memo", "LIB_PATH", "STATUS"]
+
+LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
+
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "bucket with 3000 or more keys",
+    4: "duplicate keys", 5: "seed limit exceeded", 6: "Elias-Fano offset overflow", 7: "out of memory",
+    8: "malformed serialized stream", 9: "no device copy",
+}
+
+KIND_NAMES = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforce"]
+
+
+class RecSplitError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class BuildStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double), ("partition_ms", C.c_double), ("search_ms", C.c_double * 8),
+        ("split_ms", C.c_double), ("encode_ms", C.c_double), ("evals", C.c_uint64 * 8),
+        ("nodes", C.c_uint64 * 8), ("num_buckets", C.c_uint64), ("num_nodes", C.c_uint64),
+        ("rice_bits", C.c_uint64), ("max_bucket_size", C.c_uint32), ("search_waves", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "total_ms": self.total_ms, "partition_ms": self.partition_ms, "split_ms": self.split_ms,
+            "encode_ms": self.encode_ms,
+            "search_ms": {k: self.search_ms[i] for i, k in enumerate(KIND_NAMES)},
+            "evals": {k: int(self.evals[i]) for i, k in enumerate(KIND_NAMES)},
+            "nodes": {k: int(self.nodes[i]) for i, k in enumerate(KIND_NAMES)},
+            "num_buckets": int(self.num_buckets), "num_nodes": int(self.num_nodes),
+            "rice_bits": int(self.rice_bits), "max_bucket_size": int(self.max_bucket_size),
+            "search_waves": int(self.search_waves),
+        }
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built: run `make -C gpurecsplit_amd/csrc` (hipcc, gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u64p, u32p = C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)
+    sig = {
+        "rs_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "rs_ctx_destroy": (None, [vp]),
+        "rs_build": (C.c_int, [vp, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.POINTER(vp)]),
+        "rs_build_device": (C.c_int, [vp, C.c_void_p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.POINTER(vp)]),
+        "rs_mphf_serialized_size": (C.c_size_t, [vp]),
+        "rs_mphf_serialize": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
+        "rs_mphf_deserialize": (C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.c_int, C.POINTER(vp)]),
+        "rs_mphf_query": (C.c_uint64, [vp, C.c_uint64, C.c_uint64]),
+        "rs_mphf_query_batch": (None, [vp, u64p, C.c_size_t, u64p]),
+        "rs_query_device": (C.c_int, [vp, vp, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "rs_mphf_bits": (C.c_uint64, [vp]),
+        "rs_mphf_num_keys": (C.c_uint64, [vp]),
+        "rs_mphf_stats": (C.c_int, [vp, C.POINTER(BuildStats)]),
+        "rs_mphf_free": (None, [vp]),
+        "rs_golomb_memo": (C.c_int, [C.c_uint32, u32p]),
+        "rs_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(code: int):
+    if code != 0:
+        raise RecSplitError(code, (lib.rs_last_error() or b"").decode())
+
+
+def _keys_array(keys) -> np.ndarray:
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    if k.ndim != 2 or k.shape[1] != 2:
+        raise ValueError("keys must be an (n, 2) uint64 array of (first, second) pairs")
+    return k
+
+
+def _sync_torch(t):
+    """The library runs on its own stream: finish torch's pending work on t first."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+
+
+def golomb_memo(leaf_size: int) -> np.ndarray:
+    """Golomb-Rice memo of the product (golomb << 27 | nodes << 16 | fixed bits)."""
+    out = np.zeros(3000, dtype=np.uint32)
+    _check(lib.rs_golomb_memo(leaf_size, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+class Context:
+    """A build context on one GPU (device ordinal as seen by HIP)."""
+
+    _default: dict[int, "Context"] = {}
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.rs_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        if device not in cls._default:
+            cls._default[device] = cls(device)
+        return cls._default[device]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.rs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GPURecSplit:
+    """RecSplit MPHF built on the GPU (GPURecSplit.cuh:409-842 surface).
+
+    GPURecSplit(keys, bucket_size, leaf_size=..., rotate=True, num_threads=1)
+      keys: (n, 2) uint64 host array, or a torch uint64/int64 CUDA tensor of shape
+      (n, 2) already resident on the context's GPU (no host round trip).
+    num_threads is accepted for signature parity with the reference (it sizes
+    the host sort there); construction here runs entirely on the GPU.
+    """
+
+    def __init__(self, keys=None, bucket_size: int = 100, leaf_size: int = 8, rotate: bool = True,
+                 num_threads: int = 1, ctx: Context | None = None, _handle=None):
+        self.leaf_size = leaf_size
+        self.rotate = rotate
+        self.bucket_size = bucket_size
+        self._ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+            return
+        self._ctx = ctx or Context.default()
+        h = C.c_void_p()
+        if hasattr(keys, "data_ptr") and getattr(keys, "is_cuda", False):
+            if keys.dim() != 2 or keys.shape[1] != 2 or keys.element_size() != 8 or not keys.is_contiguous():
+                raise ValueError("device keys must be a contiguous (n, 2) 64-bit tensor")
+            _sync_torch(keys)
+            _check(lib.rs_build_device(self._ctx.h, C.c_void_p(keys.data_ptr()), keys.shape[0], leaf_size,
+                                       bucket_size, int(rotate), C.byref(h)))
+        else:
+            k = _keys_array(keys)
+            _check(lib.rs_build(self._ctx.h, k.ctypes.data_as(C.POINTER(C.c_uint64)), k.shape[0], leaf_size,
+                                bucket_size, int(rotate), C.byref(h)))
+        self.h = h
+
+    # operator>> (GPURecSplit.cuh:827-841)
+    @classmethod
+    def deserialize(cls, data: bytes, leaf_size: int, rotate: bool = True) -> "GPURecSplit":
+        h = C.c_void_p()
+        _check(lib.rs_mphf_deserialize(data, len(data), leaf_size, int(rotate), C.byref(h)))
+        import struct
+        bucket = struct.unpack_from("<Q", data, 8)[0]
+        return cls(bucket_size=bucket, leaf_size=leaf_size, rotate=rotate, _handle=h)
+
+    # operator<< (GPURecSplit.cuh:817-825)
+    def serialize(self) -> bytes:
+        n = lib.rs_mphf_serialized_size(self.h)
+        buf = C.create_string_buffer(n)
+        _check(lib.rs_mphf_serialize(self.h, buf, n))
+        return buf.raw
+
+    # operator()(const hash128_t&) (AbstractParallelRecSplit.hpp:409-466)
+    def __call__(self, key) -> int:
+        first, second = key
+        return int(lib.rs_mphf_query(self.h, int(first), int(second)))
+
+    def query(self, keys) -> np.ndarray:
+        """Host queries of many keys."""
+        k = _keys_array(keys)
+        out = np.empty(k.shape[0], dtype=np.uint64)
+        lib.rs_mphf_query_batch(self.h, k.ctypes.data_as(C.POINTER(C.c_uint64)), k.shape[0],
+                                out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return out
+
+    def query_device(self, d_keys, d_out=None):
+        """Batched GPU queries: d_keys is a contiguous (n, 2) 64-bit CUDA tensor; returns
+        (or fills) an (n,) int64 CUDA tensor of results."""
+        import torch
+        if d_out is None:
+            d_out = torch.empty(d_keys.shape[0], dtype=torch.int64, device=d_keys.device)
+        _sync_torch(d_keys)
+        _check(lib.rs_query_device(self._ctx.h, self.h, C.c_void_p(d_keys.data_ptr()), d_keys.shape[0],
+                                   C.c_void_p(d_out.data_ptr())))
+        return d_out
+
+    # getBits() (AbstractParallelRecSplit.hpp:476-478), without the object-size term
+    def get_bits(self) -> int:
+        return int(lib.rs_mphf_bits(self.h))
+
+    @property
+    def num_keys(self) -> int:
+        return int(lib.rs_mphf_num_keys(self.h))
+
+    def stats(self) -> dict:
+        s = BuildStats()
+        _check(lib.rs_mphf_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.rs_mphf_free(self.h)
+            self.h = None

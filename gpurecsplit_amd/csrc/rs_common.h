@@ -1,0 +1,83 @@
+// rs_common.h -- definitions shared by the gfx950 kernels and the host engine.
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define RS_HD __host__ __device__ __forceinline__
+#else
+#define RS_HD inline
+#endif
+
+namespace rsa {
+
+// Bucket sizes are bounded by the memo table (AbstractParallelRecSplit.hpp:60).
+constexpr uint32_t kMaxBucket = 3000;
+constexpr uint32_t kMaxLeaf = 24;    // AbstractParallelRecSplit.hpp:62
+constexpr uint32_t kMaxFanout = 9;   // AbstractParallelRecSplit.hpp:63
+constexpr uint32_t kMaxDepth = 17;   // fanout-2 levels use start_seed[0..16]
+// Seeds at or above this are reported as RS_ERR_SEED_OVERFLOW; it bounds every
+// search so that each wave reaches its exit even on adversarial input.
+constexpr uint64_t kSeedLimit = uint64_t(1) << 44;
+
+// Starting seed of each tree level (AbstractParallelRecSplit.hpp:95-98): fanout-2
+// nodes use [depth], upper aggregation [17], lower aggregation [18], leaves [19].
+#define RS_START_SEEDS                                                                          \
+    {0x106393c187cae21aULL, 0x6453cec3f7376937ULL, 0x643e521ddbd2be98ULL, 0x3740c6412f6572cbULL, \
+     0x717d47562f1ce470ULL, 0x4cd6eb4c63befb7cULL, 0x9bfd8c5e18c8da73ULL, 0x082f20e10092a9a3ULL, \
+     0x2ada2ce68d21defcULL, 0xe33cb4f3e7c6466bULL, 0x3980be458c509c59ULL, 0xc466fd9584828e8cULL, \
+     0x45f0aabe1a61ede6ULL, 0xf6e7b8b33ad9b98dULL, 0x4ef95e25f4b4983dULL, 0x81175195173b92d3ULL, \
+     0x4e50927d8dd15978ULL, 0x1ea2099d1fafae7fULL, 0x425c8a06fbaaa815ULL, 0xcd4216006c74052aULL}
+constexpr int kSeedUpper = 17, kSeedLower = 18, kSeedLeaf = 19;
+
+// Node kinds; also the index into rs_build_stats::search_ms / evals / nodes.
+enum NodeKind : uint32_t { kHigher = 0, kUpper = 1, kLower = 2, kLeafRot = 3, kLeafBrute = 4, kNumKinds = 5 };
+
+// One node of a bucket's split tree, as laid out in HBM (16 B).
+struct NodeDesc {
+    uint64_t key_off;  // first key of the node in the bucket-sorted key array
+    uint32_t slot;     // global DFS-preorder index of the node's seed
+    uint16_t m;        // number of keys
+    uint8_t depth;     // fanout-2 depth (start_seed index) for kHigher
+    uint8_t kind;
+};
+
+// Search state of one node (16 B): lowest valid seed found so far and the next
+// unclaimed chunk of candidate seeds.
+struct NodeState {
+    unsigned long long result;
+    unsigned long long next_chunk;
+};
+
+// Split parameters of a leaf size (AbstractParallelRecSplit.hpp:133-148).
+struct SplitParams {
+    uint32_t leaf, lower, upper;
+};
+
+RS_HD uint32_t higher_split(const SplitParams &P, uint32_t m) {
+    return ((uint16_t)(m / 2 + P.upper - 1) / P.upper) * P.upper;
+}
+
+RS_HD uint32_t classify(const SplitParams &P, uint32_t m, bool rotate) {
+    if (m > P.upper) return kHigher;
+    if (m > P.lower) return kUpper;
+    if (m > P.leaf) return kLower;
+    return (rotate && m == P.leaf) ? kLeafRot : kLeafBrute;
+}
+
+// MurmurHash3 fmix32 of the folded 64-bit word, low 16 bits
+// (remix32 + remixAndRemap, AbstractParallelRecSplit.hpp:277-284, 327-337).
+RS_HD uint32_t fold_h16(uint64_t z) {
+    uint32_t t = (uint32_t)(z >> 32) ^ (uint32_t)z;
+    t ^= t >> 16;
+    t *= 0x85ebca6bu;
+    t ^= t >> 13;
+    t *= 0xc2b2ae35u;
+    t ^= t >> 16;
+    return t & 0xFFFFu;
+}
+
+RS_HD uint32_t remap16(uint64_t z, uint32_t n) { return (fold_h16(z) * n) >> 16; }
+
+}  // namespace rsa

@@ -1,0 +1,680 @@
+// rs_engine.hip -- host orchestration of a RecSplit build on one gfx950 GPU and
+// the build / device-query entry points of the C ABI.
+//
+// Pipeline of one build (all on one HIP stream; three small device->host reads):
+//   1. bucket histogram, per-bucket node counts, size histogram, scans
+//      -> read the size histogram: reject buckets >= 3000 keys, size the node lists
+//   2. scatter `second` by bucket, sort each bucket in LDS (detects equal keys)
+//      -> read the error flag
+//   3. expand every bucket's split tree into per-phase node lists
+//   4. per phase (fanout-2 depth 0, 1, ..., upper aggregation, lower aggregation,
+//      rotation leaves, brute-force leaves): persistent seed-search kernel, then
+//      partition of the keys between the children
+//   5. per-bucket descriptor lengths, scan -> read the total bit count
+//   6. Golomb-Rice write; Elias-Fano scalars -> read them; Elias-Fano fill
+//   7. copy the structure to the host (and keep a device copy for queries)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rs_internal.h"
+#include "rs_kernels.hip"
+
+namespace rsa {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const char *what, int line)
+        : std::runtime_error(std::string("HIP error ") + hipGetErrorString(c) + " at rs_engine.hip:" +
+                             std::to_string(line) + ": " + what),
+          code(c) {}
+};
+struct BuildError : std::runtime_error {
+    int code;
+    BuildError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(x)                                        \
+    do {                                                    \
+        hipError_t e_ = (x);                                \
+        if (e_ != hipSuccess) throw HipError(e_, #x, __LINE__); \
+    } while (0)
+
+// Grow-only device allocation reused across builds of a context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T *get(size_t count) {
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        if (bytes > cap) {
+            if (p) HIP_CHECK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            HIP_CHECK(hipMalloc(&p, bytes));
+            cap = bytes;
+        }
+        return (T *)p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct DeviceCopy {
+    int device = 0;
+    uint64_t *rice = nullptr, *lower = nullptr, *upper_c = nullptr, *upper_p = nullptr, *jump = nullptr;
+    uint32_t *memo = nullptr;
+};
+
+void device_copy_free(DeviceCopy *d) {
+    if (!d) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(d->device);
+    for (void *p : {(void *)d->rice, (void *)d->lower, (void *)d->upper_c, (void *)d->upper_p, (void *)d->jump,
+                    (void *)d->memo})
+        if (p) (void)hipFree(p);
+    (void)hipSetDevice(prev);
+    delete d;
+}
+
+constexpr int kNumLists = kMaxDepth + 4;  // fanout-2 depths, then upper, lower, rotation leaves, brute leaves
+constexpr int kScanItems = 2048;          // items per workgroup of the scan
+
+// ------------------------------------------------------------------ scans
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scan_blocks(const T *__restrict__ in, uint64_t n, uint64_t *__restrict__ out,
+                                                     uint64_t *__restrict__ block_sums) {
+    // exclusive scan of in[0..n) extended by one zero (out has n+1 entries)
+    __shared__ uint64_t s_warp[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanItems + threadIdx.x * 8;
+    uint64_t v[8], run = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint64_t i = base + j;
+        v[j] = i < n ? (uint64_t)in[i] : 0;
+        run += v[j];
+    }
+    // wave inclusive scan of thread totals
+    uint64_t inc = run;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t t = bcast64_up(inc, o);
+        if (lane >= (uint32_t)o) inc += t;
+    }
+    if (lane == 63) s_warp[w] = inc;
+    __syncthreads();
+    uint64_t woff = 0;
+    for (uint32_t k = 0; k < w; k++) woff += s_warp[k];
+    uint64_t ex = woff + inc - run;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint64_t i = base + j;
+        if (i <= n) out[i] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == 255) block_sums[blockIdx.x] = woff + inc;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint64_t *__restrict__ out, uint64_t n,
+                                                  const uint64_t *__restrict__ offs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) out[i] += offs[i / kScanItems];
+}
+
+// ------------------------------------------------------------------ census
+
+// Node counts per phase list for one bucket-size histogram (tree shapes follow
+// AbstractParallelRecSplit.hpp:137-148).
+struct Census {
+    uint64_t count[kNumLists] = {};
+    uint32_t max_depth = 0;
+};
+
+static void census_node(const SplitParams &P, bool rotate, uint32_t s, uint32_t depth, uint64_t mult, Census &c,
+                        std::vector<std::array<uint64_t, 4>> &small, std::vector<bool> &small_done) {
+    const uint32_t kind = classify(P, s, rotate);
+    if (kind != kHigher) {
+        if (!small_done[s]) {
+            // subtree of an aggregation node or leaf: independent of depth
+            std::array<uint64_t, 4> acc{};
+            acc[kind - 1] += 1;
+            if (s > P.leaf) {
+                const uint32_t unit = kind == kUpper ? P.lower : P.leaf;
+                const uint32_t fan = (uint16_t)(s + unit - 1) / unit;
+                for (uint32_t i = 0; i < fan; i++) {
+                    const uint32_t cs = (i + 1 == fan) ? s - (fan - 1) * unit : unit;
+                    if (cs > 1) {
+                        Census sub;
+                        census_node(P, rotate, cs, 0, 1, sub, small, small_done);
+                        for (int k = 0; k < 4; k++) acc[k] += sub.count[kMaxDepth + k];
+                    }
+                }
+            }
+            small[s] = acc;
+            small_done[s] = true;
+        }
+        for (int k = 0; k < 4; k++) c.count[kMaxDepth + k] += small[s][k] * mult;
+        return;
+    }
+    if (depth >= kMaxDepth) throw BuildError(RS_ERR_ARG, "split tree deeper than the start-seed table");
+    c.count[depth] += mult;
+    c.max_depth = std::max(c.max_depth, depth);
+    const uint32_t split = higher_split(P, s);
+    census_node(P, rotate, split, depth + 1, mult, c, small, small_done);
+    if (s - split > 1) census_node(P, rotate, s - split, depth + 1, mult, c, small, small_done);
+}
+
+}  // namespace rsa
+
+using namespace rsa;
+
+struct rs_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
+        list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, rice, ef_scalars, ef_lower, ef_uc, ef_up,
+        ef_jump;
+    uint32_t *memo[kMaxLeaf + 1] = {};
+    uint32_t *h_pinned = nullptr;  // small pinned staging area
+    hipEvent_t ev[128];
+    int n_ev = 0;
+};
+
+namespace rsa {
+
+static void scan_u64(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out) {
+    // out[0..n] = exclusive scan of in[0..n), out[n] = total
+    const uint64_t items = n + 1;
+    const uint64_t nblk = (items + kScanItems - 1) / kScanItems;
+    // temp: block sums + their scan (recursive levels are small)
+    uint64_t *tmp = c->scan_tmp.get<uint64_t>(4 * (nblk + 2) + 64);
+    uint64_t *sums = tmp, *offs = tmp + nblk + 1;
+    if (in32)
+        k_scan_blocks<uint32_t><<<nblk, 256, 0, c->stream>>>((const uint32_t *)in, n, out, sums);
+    else
+        k_scan_blocks<uint64_t><<<nblk, 256, 0, c->stream>>>((const uint64_t *)in, n, out, sums);
+    HIP_CHECK(hipGetLastError());
+    if (nblk > 1) {
+        if (nblk + 1 > kScanItems) {
+            // second level larger than one workgroup: scan it on the host (rare: > 4M items)
+            std::vector<uint64_t> h(nblk);
+            HIP_CHECK(hipMemcpyAsync(h.data(), sums, nblk * 8, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            uint64_t run = 0;
+            for (auto &x : h) {
+                const uint64_t t = x;
+                x = run;
+                run += t;
+            }
+            HIP_CHECK(hipMemcpyAsync(offs, h.data(), nblk * 8, hipMemcpyHostToDevice, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        } else {
+            k_scan_blocks<uint64_t><<<1, 256, 0, c->stream>>>(sums, nblk, offs, offs + nblk + 1);
+            HIP_CHECK(hipGetLastError());
+        }
+        k_scan_add<<<(items + 255) / 256, 256, 0, c->stream>>>(out, n, offs);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
+static const uint32_t *ctx_memo(rs_ctx *c, uint32_t leaf) {
+    if (!c->memo[leaf]) {
+        const Memo &m = golomb_memo(leaf);
+        HIP_CHECK(hipMalloc(&c->memo[leaf], sizeof(uint32_t) * kMaxBucket));
+        HIP_CHECK(hipMemcpy(c->memo[leaf], m.data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice));
+    }
+    return c->memo[leaf];
+}
+
+// Chooses the packed-counter layout of an aggregation level: the narrowest field
+// width with 2^fb >= split + 2; 32-bit counters when (fanout-1) fields fit.
+struct AggrLayout {
+    uint32_t fb, magic;
+    bool wide;
+};
+static AggrLayout aggr_layout(uint32_t split, uint32_t max_fanout) {
+    uint32_t fb = 1;
+    while ((1u << fb) < split + 2) fb++;
+    AggrLayout L{fb, ((1u << 20) + split - 1) / split, false};
+    if (fb * (max_fanout - 1) > 31) L.wide = true;
+    if (fb * (max_fanout - 1) > 63) throw BuildError(RS_ERR_ARG, "aggregation counters do not fit 64 bits");
+    // (r * magic) >> 20 == r / split for every remapped value r < 3000
+    for (uint32_t r = 0; r < kMaxBucket; r++)
+        if (((r * L.magic) >> 20) != r / split) throw BuildError(RS_ERR_ARG, "aggregation divisor not exact");
+    return L;
+}
+
+static int ev_record(rs_ctx *c) {
+    if (c->n_ev >= 128) return -1;
+    HIP_CHECK(hipEventRecord(c->ev[c->n_ev], c->stream));
+    return c->n_ev++;
+}
+static double ev_ms(rs_ctx *c, int a, int b) {
+    if (a < 0 || b < 0) return 0;
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
+    return ms;
+}
+
+static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, SearchArgs a, bool wide, uint32_t blocks) {
+    constexpr int U = 2;
+    hipStream_t s = c->stream;
+    switch (kind) {
+    case kHigher: k_search<kHigher, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
+    case kUpper:
+        if (wide) k_search<kUpper, U, true><<<blocks, 256, 0, s>>>(keys, a);
+        else k_search<kUpper, U, false><<<blocks, 256, 0, s>>>(keys, a);
+        break;
+    case kLower:
+        if (wide) k_search<kLower, U, true><<<blocks, 256, 0, s>>>(keys, a);
+        else k_search<kLower, U, false><<<blocks, 256, 0, s>>>(keys, a);
+        break;
+    case kLeafRot: k_search<kLeafRot, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
+    default: k_search<kLeafBrute, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t leaf, uint64_t bucket_size,
+                      bool rotate, const std::chrono::steady_clock::time_point t_start) {
+    hipStream_t s = c->stream;
+    c->n_ev = 0;
+    const SplitParams P = split_params(leaf);
+    const uint32_t *memo = ctx_memo(c, leaf);
+    const Memo &hmemo = golomb_memo(leaf);
+    const uint64_t nb = std::max<uint64_t>(1, (n + bucket_size - 1) / bucket_size);
+    rs_build_stats st{};
+    st.num_buckets = nb;
+
+    // ---- 1. bucket histogram and per-bucket metadata
+    const int e_part0 = ev_record(c);
+    uint32_t *counts = c->counts.get<uint32_t>(nb);
+    uint64_t *counts64 = c->counts64.get<uint64_t>(nb);
+    uint64_t *acc = c->acc.get<uint64_t>(nb + 1);
+    uint64_t *node_cnt = c->node_cnt.get<uint64_t>(nb);
+    uint64_t *node_base = c->node_base.get<uint64_t>(nb + 1);
+    uint32_t *size_hist = c->size_hist.get<uint32_t>(kMaxBucket + 1);
+    uint32_t *err = c->err.get<uint32_t>(4);
+    HIP_CHECK(hipMemsetAsync(counts, 0, nb * 4, s));
+    HIP_CHECK(hipMemsetAsync(size_hist, 0, (kMaxBucket + 1) * 4, s));
+    HIP_CHECK(hipMemsetAsync(err, 0, 16, s));
+    const uint32_t hist_blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c->num_cus * 8);
+    if (nb <= 8192)
+        k_bucket_hist<true><<<hist_blocks, 256, nb * 4, s>>>(d_keys, n, nb, counts);
+    else
+        k_bucket_hist<false><<<hist_blocks, 256, 0, s>>>(d_keys, n, nb, counts);
+    HIP_CHECK(hipGetLastError());
+    k_bucket_meta<<<(nb + 255) / 256, 256, 0, s>>>(counts, nb, memo, counts64, node_cnt, size_hist);
+    HIP_CHECK(hipGetLastError());
+    scan_u64(c, counts, true, nb, acc);
+    scan_u64(c, node_cnt, false, nb, node_base);
+    uint32_t *h = c->h_pinned;
+    HIP_CHECK(hipMemcpyAsync(h, size_hist, (kMaxBucket + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h[kMaxBucket]) throw BuildError(RS_ERR_BUCKET_TOO_LARGE, "a bucket holds 3000 or more keys");
+    Census cen;
+    {
+        std::vector<std::array<uint64_t, 4>> small(kMaxBucket);
+        std::vector<bool> small_done(kMaxBucket, false);
+        for (uint32_t m = 2; m < kMaxBucket; m++) {
+            if (!h[m]) continue;
+            st.max_bucket_size = m;
+            census_node(P, rotate, m, 0, h[m], cen, small, small_done);
+        }
+        if (!st.max_bucket_size) st.max_bucket_size = h[1] ? 1 : 0;
+    }
+    uint64_t total_nodes = 0;
+    std::vector<uint64_t> lbase(kNumLists);
+    for (int l = 0; l < kNumLists; l++) {
+        lbase[l] = total_nodes;
+        total_nodes += cen.count[l];
+    }
+    st.num_nodes = total_nodes;
+    for (int l = 0; l < kNumLists; l++) {
+        const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
+        st.nodes[kind] += cen.count[l];
+    }
+
+    // ---- 2. scatter and per-bucket sort
+    uint64_t *sorted = c->sorted.get<uint64_t>(n);
+    uint32_t *cursor = c->cursor.get<uint32_t>(nb);
+    HIP_CHECK(hipMemsetAsync(cursor, 0, nb * 4, s));
+    k_scatter<<<hist_blocks, 256, 0, s>>>(d_keys, n, nb, acc, cursor, sorted);
+    HIP_CHECK(hipGetLastError());
+    k_sort_buckets<<<nb, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err);
+    HIP_CHECK(hipGetLastError());
+
+    // ---- 3. node lists
+    const uint64_t pool_n = std::max<uint64_t>(total_nodes, 1);
+    NodeDesc *pool = c->pool.get<NodeDesc>(pool_n);
+    uint64_t *d_lbase = c->list_base.get<uint64_t>(kNumLists);
+    uint32_t *d_lfill = c->list_fill.get<uint32_t>(kNumLists);
+    uint16_t *node_m = c->node_m.get<uint16_t>(pool_n);
+    uint64_t *seeds = c->seeds.get<uint64_t>(pool_n);
+    HIP_CHECK(hipMemcpyAsync(d_lbase, lbase.data(), kNumLists * 8, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(d_lfill, 0, kNumLists * 4, s));
+    ExpandArgs ea{P, rotate ? 1 : 0, pool, d_lbase, d_lfill, node_m};
+    k_expand<<<(nb + 63) / 64, 64, 0, s>>>(acc, node_base, nb, ea);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h[0] & 1) throw BuildError(RS_ERR_DUPLICATE_KEYS, "two keys of one bucket share the same `second` half");
+    const int e_part1 = ev_record(c);
+
+    // ---- 4. search phases
+    uint64_t max_list = 1;
+    for (int l = 0; l < kNumLists; l++) max_list = std::max(max_list, cen.count[l]);
+    NodeState *state = c->state.get<NodeState>(max_list);
+    const uint32_t blocks = (uint32_t)c->num_cus * 8;
+    const uint32_t n_waves = blocks * 4;
+    st.search_waves = n_waves;
+    int *running = c->running.get<int>(n_waves);
+    uint32_t *ctl = c->ctl.get<uint32_t>(16);
+    const AggrLayout up_l = aggr_layout(P.lower, P.upper / P.lower);
+    const AggrLayout lo_l = aggr_layout(P.leaf, P.lower / P.leaf);
+    struct PhaseEvents {
+        uint32_t kind;
+        int search0, search1, split1;
+    };
+    std::vector<PhaseEvents> phase_ev;
+    for (int l = 0; l < kNumLists; l++) {
+        const uint32_t cnt = (uint32_t)cen.count[l];
+        if (!cnt) continue;
+        const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
+        const NodeDesc *nodes = pool + lbase[l];
+        k_state_init<<<(cnt + 255) / 256, 256, 0, s>>>(state, cnt);
+        HIP_CHECK(hipMemsetAsync(running, 0xFF, n_waves * 4, s));
+        HIP_CHECK(hipMemsetAsync(ctl, 0, 64, s));
+        SearchArgs a{};
+        a.nodes = nodes;
+        a.state = state;
+        a.running = running;
+        a.ctl = ctl;
+        a.n_nodes = cnt;
+        a.n_waves = n_waves;
+        a.leaf = P.leaf;
+        a.upper = P.upper;
+        bool wide = false;
+        switch (kind) {
+        case kHigher: a.groups = 1; break;
+        case kUpper:
+            a.groups = 1;
+            a.split_unit = P.lower;
+            a.field_bits = up_l.fb;
+            a.magic = up_l.magic;
+            wide = up_l.wide;
+            break;
+        case kLower:
+            a.groups = 2;
+            a.split_unit = P.leaf;
+            a.field_bits = lo_l.fb;
+            a.magic = lo_l.magic;
+            wide = lo_l.wide;
+            break;
+        default: a.groups = 4; break;
+        }
+        PhaseEvents pe{kind, ev_record(c), -1, -1};
+        launch_search(c, kind, sorted, a, wide, blocks);
+        pe.search1 = ev_record(c);
+        if (kind == kHigher || kind == kUpper || kind == kLower) {
+            const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);
+            k_split<<<cnt, 64, (size_t)max_m * 8, s>>>(sorted, nodes, state, seeds, err, P, kind);
+        } else {
+            k_finalize<<<(cnt + 255) / 256, 256, 0, s>>>(nodes, state, cnt, seeds, err);
+        }
+        HIP_CHECK(hipGetLastError());
+        pe.split1 = ev_record(c);
+        phase_ev.push_back(pe);
+    }
+    const int e_search_end = ev_record(c);
+
+    // ---- 5. descriptor lengths
+    uint64_t *bits = c->bits.get<uint64_t>(nb);
+    uint64_t *pos = c->pos.get<uint64_t>(nb + 1);
+    unsigned long long *evals = c->evals.get<unsigned long long>(kNumKinds);
+    HIP_CHECK(hipMemsetAsync(evals, 0, kNumKinds * 8, s));
+    EncodeArgs ca{acc, node_base, seeds, node_m, memo, nb, P, rotate ? 1 : 0};
+    k_bucket_bits<<<(nb + 3) / 4, 256, 0, s>>>(ca, bits, evals);
+    HIP_CHECK(hipGetLastError());
+    scan_u64(c, bits, false, nb, pos);
+    uint64_t *h64 = (uint64_t *)c->h_pinned;
+    HIP_CHECK(hipMemcpyAsync(h64, pos + nb, 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h64 + 1, evals, kNumKinds * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    const uint64_t total_bits = h64[0];
+    for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
+    if (h[16] & 2) throw BuildError(RS_ERR_SEED_OVERFLOW, "a node search passed the 2^44 seed limit");
+    const uint64_t B = total_bits + 1;  // with the sentinel
+    const uint64_t rice_words = ((B + 7) / 8 + 14) / 8;
+    st.rice_bits = B;
+
+    // ---- 6. Golomb-Rice and Elias-Fano
+    unsigned long long *rice = c->rice.get<unsigned long long>(rice_words + 2);
+    HIP_CHECK(hipMemsetAsync(rice, 0, (rice_words + 2) * 8, s));
+    k_rice_write<<<(nb + 3) / 4, 256, 0, s>>>(ca, pos, rice);
+    k_rice_sentinel<<<1, 1, 0, s>>>(rice, pos, nb);
+    EfScalars *efs = c->ef_scalars.get<EfScalars>(1);
+    k_ef_scalars<<<1, 1024, 0, s>>>(acc, pos, nb, efs);
+    HIP_CHECK(hipGetLastError());
+    EfScalars he;
+    HIP_CHECK(hipMemcpyAsync(h64, efs, sizeof(EfScalars), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(&he, h64, sizeof(he));
+    EliasFano ef;
+    ef.num_buckets = nb;
+    ef.bits_per_key_fixed_point = he.bpk;
+    ef.cum_keys_min_delta = he.min_delta;
+    ef.min_diff = he.min_diff;
+    // DoubleEF.hpp:181-184
+    ef.u_position = (uint64_t)((int64_t)total_bits - (int64_t)(he.bpk * n >> 20) - (int64_t)(nb * (uint64_t)he.min_diff) + 1);
+    ef.u_cum_keys = n - nb * (uint64_t)he.min_delta + 1;
+    ef.derive_widths();
+    const uint64_t n_lower = ef_lower_words(nb, ef.l_cum_keys, ef.l_position);
+    const uint64_t n_uc = ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys);
+    const uint64_t n_up = ef_upper_words(nb, ef.u_position, ef.l_position);
+    const uint64_t n_jump = ef_jump_words(nb);
+    unsigned long long *d_lower = c->ef_lower.get<unsigned long long>(n_lower + 1);
+    unsigned long long *d_uc = c->ef_uc.get<unsigned long long>(n_uc + 1);
+    unsigned long long *d_up = c->ef_up.get<unsigned long long>(n_up + 1);
+    uint8_t *d_jump = c->ef_jump.get<uint8_t>(n_jump * 8 + 8);
+    HIP_CHECK(hipMemsetAsync(d_lower, 0, (n_lower + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_uc, 0, (n_uc + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_up, 0, (n_up + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_jump, 0, n_jump * 8 + 8, s));
+    EfFill ff{acc, pos, nb, he.bpk, ef.l_cum_keys, ef.l_position, he.min_delta, he.min_diff, d_lower, d_uc, d_up,
+              d_jump, n_jump * 8, err};
+    k_ef_fill<<<(nb + 1 + 255) / 256, 256, 0, s>>>(ff);
+    HIP_CHECK(hipGetLastError());
+    const int e_enc_end = ev_record(c);
+
+    // ---- 7. results to the host
+    rs_mphf *out = new rs_mphf();
+    out->leaf = leaf;
+    out->rotate = rotate ? 1 : 0;
+    out->bucket_size = bucket_size;
+    out->keys_count = n;
+    out->nbuckets = nb;
+    out->rice.assign(rice_words + 2, 0);
+    out->rice_words = rice_words;
+    ef.lower_bits.assign(n_lower, 0);
+    ef.upper_cum.assign(n_uc, 0);
+    ef.upper_pos.assign(n_up, 0);
+    ef.jump.assign(n_jump, 0);
+    HIP_CHECK(hipMemcpyAsync(out->rice.data(), rice, rice_words * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ef.lower_bits.data(), d_lower, n_lower * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ef.upper_cum.data(), d_uc, n_uc * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ef.upper_pos.data(), d_up, n_up * 8, hipMemcpyDeviceToHost, s));
+    if (n_jump) HIP_CHECK(hipMemcpyAsync(ef.jump.data(), d_jump, n_jump * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
+    // device copy for batched queries
+    DeviceCopy *dc = new DeviceCopy();
+    dc->device = c->device;
+    out->dev = dc;
+    HIP_CHECK(hipMalloc(&dc->rice, (rice_words + 2) * 8));
+    HIP_CHECK(hipMalloc(&dc->lower, (n_lower + 1) * 8));
+    HIP_CHECK(hipMalloc(&dc->upper_c, (n_uc + 1) * 8));
+    HIP_CHECK(hipMalloc(&dc->upper_p, (n_up + 1) * 8));
+    HIP_CHECK(hipMalloc(&dc->jump, n_jump * 8 + 8));
+    HIP_CHECK(hipMalloc(&dc->memo, kMaxBucket * 4));
+    HIP_CHECK(hipMemcpyAsync(dc->rice, rice, (rice_words + 2) * 8, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dc->lower, d_lower, (n_lower + 1) * 8, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dc->upper_c, d_uc, (n_uc + 1) * 8, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (n_up + 1) * 8, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, n_jump * 8 + 8, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(dc->memo, memo, kMaxBucket * 4, hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (h[16] & 4) {
+        rs_mphf_free(out);
+        throw BuildError(RS_ERR_EF_OVERFLOW, "Elias-Fano jump offset does not fit 16 bits");
+    }
+    out->ef = std::move(ef);
+    out->memo = &hmemo;
+    out->P = P;
+
+    // ---- timings
+    st.partition_ms = ev_ms(c, e_part0, e_part1);
+    for (const PhaseEvents &pe : phase_ev) {
+        st.search_ms[pe.kind] += ev_ms(c, pe.search0, pe.search1);
+        st.split_ms += ev_ms(c, pe.search1, pe.split1);
+    }
+    st.encode_ms = ev_ms(c, e_search_end, e_enc_end);
+    st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    out->stats = st;
+    return out;
+}
+
+static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn) {
+    try {
+        std::lock_guard<std::mutex> g(c->mu);
+        HIP_CHECK(hipSetDevice(c->device));
+        fn();
+        return RS_OK;
+    } catch (const BuildError &e) {
+        set_error(std::string(what) + ": " + e.what());
+        return e.code;
+    } catch (const HipError &e) {
+        set_error(std::string(what) + ": " + e.what());
+        return RS_ERR_HIP;
+    } catch (const std::bad_alloc &) {
+        set_error(std::string(what) + ": out of host memory");
+        return RS_ERR_NOMEM;
+    }
+}
+
+static int check_args(rs_ctx *c, const void *keys, size_t n, uint32_t leaf, size_t bucket, rs_mphf **out) {
+    if (!c || !keys || !out) return set_error("null argument"), RS_ERR_ARG;
+    if (n == 0) return set_error("no keys"), RS_ERR_ARG;
+    if (leaf < 2 || leaf > kMaxLeaf) return set_error("leaf size must be in [2, 24]"), RS_ERR_ARG;
+    if (bucket == 0) return set_error("bucket size must be positive"), RS_ERR_ARG;
+    if (n >= (uint64_t(1) << 40)) return set_error("more than 2^40 keys"), RS_ERR_ARG;
+    return RS_OK;
+}
+
+}  // namespace rsa
+
+extern "C" {
+
+int rs_ctx_create(int device, rs_ctx **out) {
+    if (!out) return RS_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        set_error("rs_ctx_create: no such HIP device");
+        return RS_ERR_HIP;
+    }
+    rs_ctx *c = new rs_ctx();
+    c->device = device;
+    try {
+        HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount;
+        HIP_CHECK(hipStreamCreate(&c->stream));  // blocking: ordered after null-stream work
+        HIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * 1024, hipHostMallocDefault));
+        for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    } catch (const HipError &e) {
+        set_error(std::string("rs_ctx_create: ") + e.what());
+        delete c;
+        return RS_ERR_HIP;
+    }
+    *out = c;
+    return RS_OK;
+}
+
+void rs_ctx_destroy(rs_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->keys_in, &c->counts, &c->counts64, &c->cursor, &c->acc, &c->node_cnt, &c->node_base,
+                      &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
+                      &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->rice,
+                      &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump})
+        b->release();
+    for (auto &m : c->memo)
+        if (m) (void)hipFree(m);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int rs_build_device(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_t leaf, size_t bucket, int rotate,
+                    rs_mphf **out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (int e = check_args(c, d_keys, n, leaf, bucket, out)) return e;
+    return guarded("rs_build_device", c, [&] { *out = build(c, d_keys, n, leaf, bucket, rotate != 0, t0); });
+}
+
+int rs_build(rs_ctx *c, const uint64_t *keys, size_t n, uint32_t leaf, size_t bucket, int rotate, rs_mphf **out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (int e = check_args(c, keys, n, leaf, bucket, out)) return e;
+    return guarded("rs_build", c, [&] {
+        uint64_t *d = c->keys_in.get<uint64_t>(2 * n);
+        HIP_CHECK(hipMemcpyAsync(d, keys, n * 16, hipMemcpyHostToDevice, c->stream));
+        *out = build(c, d, n, leaf, bucket, rotate != 0, t0);
+    });
+}
+
+int rs_query_device(rs_ctx *c, const rs_mphf *h, const uint64_t *d_keys, size_t n, uint64_t *d_out) {
+    if (!c || !h || (!d_keys && n) || (!d_out && n)) return set_error("rs_query_device: null argument"), RS_ERR_ARG;
+    if (!h->dev || h->dev->device != c->device)
+        return set_error("rs_query_device: structure has no copy on this device"), RS_ERR_NO_DEVICE;
+    if (n == 0) return RS_OK;
+    return guarded("rs_query_device", c, [&] {
+        QueryArgs q{};
+        q.rice = h->dev->rice;
+        q.lower = h->dev->lower;
+        q.upper_c = h->dev->upper_c;
+        q.upper_p = h->dev->upper_p;
+        q.jump = h->dev->jump;
+        q.memo = h->dev->memo;
+        q.nbuckets = h->nbuckets;
+        q.bpk = h->ef.bits_per_key_fixed_point;
+        q.lc = h->ef.l_cum_keys;
+        q.lp = h->ef.l_position;
+        q.min_delta = h->ef.cum_keys_min_delta;
+        q.min_diff = h->ef.min_diff;
+        q.P = h->P;
+        q.rotate = h->rotate;
+        k_query<<<(n + 255) / 256, 256, 0, c->stream>>>(q, d_keys, n, d_out);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,929 @@
+// rs_kernels.hip -- gfx950 kernels of the RecSplit construction engine.
+//
+// Data layout in HBM (one build):
+//   keys_in   n x 16 B   (first, second) pairs, read twice (histogram, scatter)
+//   sorted    n x 8 B    `second` of every key, grouped by bucket; inside a bucket
+//                        sorted ascending, then permuted in place as the split
+//                        tree is descended (each node's keys stay contiguous)
+//   nodes     N x 16 B   NodeDesc of every tree node, grouped by search phase
+//   state     16 B/node  NodeState of the phase being searched
+//   seeds     N x 8 B    final seed of every node, by DFS-preorder slot
+//   node_m    N x 2 B    size of every node, by slot
+//
+// Seed search (the hot loop): a persistent kernel per phase.  Each wavefront
+// owns one node at a time, claims chunks of candidate seeds from the node's
+// NodeState with one atomic, and evaluates them with 64 lanes x U seeds in
+// flight per lane while streaming the node's keys through scalar registers
+// (wave-uniform loads).  A ballot picks the lowest valid seed of a group; the
+// node's result is the atomic minimum over all chunks, and a chunk is only
+// skipped once its first seed is not below that minimum, so the result is the
+// lowest valid seed.  Waves with no fresh node help the nodes still running.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "rs_common.h"
+
+namespace rsa {
+
+__constant__ uint64_t c_start_seed[20] = RS_START_SEEDS;
+
+constexpr unsigned long long kNone = ~0ull;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-uniform values, kept in scalar registers.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+    const uint32_t lo = __shfl((uint32_t)v, src), hi = __shfl((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t bcast64_xor(uint64_t v, int mask) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, mask), hi = __shfl_xor((uint32_t)(v >> 32), mask);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t bcast64_up(uint64_t v, int delta) {
+    const uint32_t lo = __shfl_up((uint32_t)v, delta), hi = __shfl_up((uint32_t)(v >> 32), delta);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Elias-Fano geometry (DoubleEF.hpp:63-69): q = 256 elements per sample,
+// super blocks of 2^14 elements, 17 words per super block and sequence.
+constexpr uint64_t kEfQMaskDev = 255;
+constexpr uint64_t kEfSuperSizeDev = 17;
+
+__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------ partition
+
+// Bucket histogram; bucket = floor(first * nb / 2^64) (common.hpp:273-280).
+template <bool PRIV>
+__global__ __launch_bounds__(256) void k_bucket_hist(const uint64_t *__restrict__ keys, uint64_t n, uint64_t nb,
+                                                     uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t s_cnt[];
+    if (PRIV) {
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_cnt[i] = 0;
+        __syncthreads();
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = __umul64hi(keys[2 * i], nb);
+        if (PRIV) atomicAdd(&s_cnt[b], 1u);
+        else atomicAdd(&counts[b], 1u);
+    }
+    if (PRIV) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+            if (s_cnt[i]) atomicAdd(&counts[i], s_cnt[i]);
+    }
+}
+
+// Per bucket: node count (memo), size histogram, oversize flag.
+__global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict__ counts, uint64_t nb,
+                                                     const uint32_t *__restrict__ memo, uint64_t *__restrict__ counts64,
+                                                     uint64_t *__restrict__ nodes, uint32_t *__restrict__ size_hist) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t m = counts[b];
+    counts64[b] = m;
+    if (m >= kMaxBucket) {
+        atomicAdd(&size_hist[kMaxBucket], 1u);
+        nodes[b] = 0;
+        return;
+    }
+    atomicAdd(&size_hist[m], 1u);
+    nodes[b] = (memo[m] >> 16) & 0x7FF;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const uint64_t *__restrict__ keys, uint64_t n, uint64_t nb,
+                                                 const uint64_t *__restrict__ bucket_acc, uint32_t *__restrict__ cursor,
+                                                 uint64_t *__restrict__ sorted) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = __umul64hi(keys[2 * i], nb);
+        const uint32_t p = atomicAdd(&cursor[b], 1u);
+        sorted[bucket_acc[b] + p] = keys[2 * i + 1];
+    }
+}
+
+// One workgroup per bucket: bitonic sort of the bucket's `second` values in LDS
+// (canonical order, independent of the scatter's atomics) and detection of equal
+// values, which no seed can ever separate.
+__global__ __launch_bounds__(256) void k_sort_buckets(uint64_t *__restrict__ sorted, const uint64_t *__restrict__ acc,
+                                                      uint32_t *__restrict__ err) {
+    __shared__ uint64_t s[4096];
+    const uint64_t b = blockIdx.x;
+    const uint64_t off = acc[b];
+    const uint32_t m = (uint32_t)(acc[b + 1] - off);
+    if (m < 2 || m >= kMaxBucket) return;
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) s[i] = i < m ? sorted[off + i] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = s[i], c = s[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > c) == up) {
+                        s[i] = c;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        sorted[off + i] = s[i];
+        if (i > 0 && s[i] == s[i - 1]) atomicOr(err, 1u);
+    }
+}
+
+// One thread per bucket: walks the bucket's split tree in DFS preorder and emits
+// every node (key range, size, slot, fanout-2 depth) into its phase list.  The
+// tree shape depends only on the bucket size (AbstractParallelRecSplit.hpp:137-148,
+// SIMDRecSplit.hpp:624-693), so all nodes are known before any search.
+struct ExpandArgs {
+    SplitParams P;
+    int rotate;
+    NodeDesc *pool;
+    const uint64_t *list_base;  // [kMaxDepth + 4]
+    uint32_t *list_fill;        // [kMaxDepth + 4]
+    uint16_t *node_m;
+};
+
+__device__ __forceinline__ uint32_t list_of(uint32_t kind, uint32_t depth) {
+    return kind == kHigher ? depth : kMaxDepth + kind - 1;
+}
+
+__global__ __launch_bounds__(64) void k_expand(const uint64_t *__restrict__ acc, const uint64_t *__restrict__ node_base,
+                                               uint64_t nb, ExpandArgs a) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t m = (uint32_t)(acc[b + 1] - acc[b]);
+    if (m < 2 || m >= kMaxBucket) return;
+    struct Item {
+        uint32_t off;
+        uint16_t m;
+        uint8_t depth;
+    };
+    Item stack[48];
+    int sp = 0;
+    stack[sp++] = Item{0, (uint16_t)m, 0};
+    uint32_t slot = (uint32_t)node_base[b];
+    const SplitParams P = a.P;
+    while (sp > 0) {
+        const Item it = stack[--sp];
+        const uint32_t kind = classify(P, it.m, a.rotate != 0);
+        const uint32_t list = list_of(kind, it.depth);
+        const uint32_t pos = atomicAdd(&a.list_fill[list], 1u);
+        NodeDesc d;
+        d.key_off = acc[b] + it.off;
+        d.slot = slot;
+        d.m = it.m;
+        d.depth = it.depth;
+        d.kind = (uint8_t)kind;
+        a.pool[a.list_base[list] + pos] = d;
+        a.node_m[slot] = it.m;
+        slot++;
+        if (it.m <= P.leaf) continue;
+        uint32_t unit, fan;
+        if (kind == kHigher) {
+            unit = higher_split(P, it.m);
+            fan = 2;
+        } else if (kind == kUpper) {
+            unit = P.lower;
+            fan = (uint16_t)(it.m + P.lower - 1) / P.lower;
+        } else {
+            unit = P.leaf;
+            fan = (uint16_t)(it.m + P.leaf - 1) / P.leaf;
+        }
+        // push children right to left so that the leftmost is visited next
+        for (int c = (int)fan - 1; c >= 0; c--) {
+            const uint32_t cs = (c == (int)fan - 1) ? it.m - (fan - 1) * unit : unit;
+            if (cs > 1) stack[sp++] = Item{it.off + (uint32_t)c * unit, (uint16_t)cs, (uint8_t)(it.depth + 1)};
+        }
+    }
+}
+
+// ------------------------------------------------------------------ seed search
+
+struct SearchArgs {
+    const NodeDesc *nodes;
+    NodeState *state;
+    int *running;
+    uint32_t *ctl;  // [0] fresh-node cursor
+    uint32_t n_nodes;
+    uint32_t n_waves;
+    uint32_t groups;      // groups of 64*U candidates per claimed chunk
+    uint32_t leaf;        // leaf size
+    uint32_t split_unit;  // part size of an aggregation level
+    uint32_t field_bits;  // packed counter field width
+    uint32_t magic;       // ceil(2^20 / split_unit)
+    uint32_t upper;       // upper aggregation bound (fanout-2 split size)
+};
+
+// Fanout-2 node: lowest x with #{remap(k+S[depth]+x, m) < split} == split
+// (SIMDRecSplit.hpp:250-317).  remap < split <=> h16 < ceil(split*2^16/m).
+template <int U>
+__device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
+                                                 uint32_t split, uint32_t H, uint64_t u0) {
+    const uint32_t lane = lane_id();
+    uint64_t s[U];
+    uint32_t cnt[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        cnt[u] = 0;
+    }
+#pragma unroll 4
+    for (uint32_t i = 0; i < m; i++) {
+        const uint64_t k = kp[i] + sconst;
+#pragma unroll
+        for (int u = 0; u < U; u++) cnt[u] += fold_h16(k + s[u]) < H;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const unsigned long long msk = __ballot(cnt[u] == split);
+        if (msk) return u0 + (uint64_t)(u * 64 + __ffsll((long long)msk) - 1);
+    }
+    return kNone;
+}
+
+// Aggregation node: lowest x whose parts floor(remap/SPLIT) have exactly the
+// expected counts (SIMDRecSplit.hpp:319-423).  Counts are packed in fields of
+// `fb` bits (2^fb >= SPLIT + 2), the last part uncounted; a carry out of any field
+// forces another count negative, so equality of the counted fields is
+// equality of all counts.
+template <int U, bool WIDE>
+__device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
+                                               uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask, uint64_t u0) {
+    using C = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    const uint32_t lane = lane_id();
+    uint64_t s[U];
+    C cnt[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        cnt[u] = 0;
+    }
+#pragma unroll 4
+    for (uint32_t i = 0; i < m; i++) {
+        const uint64_t k = kp[i] + sconst;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t r = __umul24(fold_h16(k + s[u]), m) >> 16;
+            const uint32_t part = __umul24(r, magic) >> 20;
+            cnt[u] += (C)1 << __umul24(part, fb);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const unsigned long long msk = __ballot(((uint64_t)cnt[u] & cmask) == T);
+        if (msk) return u0 + (uint64_t)(u * 64 + __ffsll((long long)msk) - 1);
+    }
+    return kNone;
+}
+
+// Full leaf with rotation fitting (SIMDRecSplit.hpp:452-509): unit j is the base
+// seed j*L; keys with even `second` set bits of the left mask, odd ones of the
+// right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
+template <int U>
+__device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
+                                                   uint32_t nr, uint64_t u0) {
+    const uint32_t lane = lane_id();
+    const uint64_t S = c_start_seed[kSeedLeaf];
+    const uint32_t full = (uint32_t)((1ull << L) - 1);
+    uint64_t sb[U];
+    uint32_t ml[U], mr[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        sb[u] = (u0 + (uint64_t)(u * 64 + lane)) * L;
+        ml[u] = mr[u] = 0;
+    }
+    for (uint32_t i = 0; i < L; i++) {
+        const uint64_t key = kp[i];
+        const uint64_t k = key + S;
+        if (key & 1) {
+#pragma unroll
+            for (int u = 0; u < U; u++) mr[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; u++) ml[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
+        if (!__ballot(ok)) continue;
+        uint32_t rf = 0xFFFFFFFFu;
+        if (ok) {
+            for (uint32_t r = 0; r < L; r++) {
+                const uint32_t rot = r == 0 ? mr[u] : (((mr[u] << r) | (mr[u] >> (L - r))) & full);
+                if ((ml[u] | rot) == full) {
+                    rf = r;
+                    break;
+                }
+            }
+        }
+        const unsigned long long msk = __ballot(rf != 0xFFFFFFFFu);
+        if (msk) {
+            const int l = __ffsll((long long)msk) - 1;
+            const uint32_t r = uni(__shfl(rf, l));
+            return (u0 + (uint64_t)(u * 64 + l)) * L + r;
+        }
+    }
+    return kNone;
+}
+
+// Leaf without rotation (SIMDRecSplit.hpp:510-599): lowest x whose m keys land on
+// m distinct positions.  A prefix of ceil(3 sqrt(m)) keys must already be
+// collision-free; waves skip the rest when no lane passes (lossless).
+template <int U>
+__device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict__ kp, uint32_t m, uint32_t mid,
+                                                     uint64_t u0) {
+    const uint32_t lane = lane_id();
+    const uint64_t S = c_start_seed[kSeedLeaf];
+    const uint32_t full = (uint32_t)((1ull << m) - 1);
+    uint64_t s[U];
+    uint32_t mk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        mk[u] = 0;
+    }
+    for (uint32_t i = 0; i < mid; i++) {
+        const uint64_t k = kp[i] + S;
+#pragma unroll
+        for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
+    }
+    if (mid < m) {
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < U; u++) any |= (uint32_t)__popc(mk[u]) == mid;
+        if (!__ballot(any)) return kNone;
+        for (uint32_t i = mid; i < m; i++) {
+            const uint64_t k = kp[i] + S;
+#pragma unroll
+            for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const unsigned long long msk = __ballot(mk[u] == full);
+        if (msk) return u0 + (uint64_t)(u * 64 + __ffsll((long long)msk) - 1);
+    }
+    return kNone;
+}
+
+__device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per_chunk, uint64_t scale) {
+    const unsigned long long nc = ld_relaxed(&st->next_chunk);
+    const unsigned long long res = ld_relaxed(&st->result);
+    return nc * units_per_chunk * scale < res;
+}
+
+template <int KIND, int U, bool WIDE>
+__global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
+    const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
+    int node = -1;
+    bool helping = false;
+    uint32_t scan = wave;
+    // cached per-node values (wave-uniform)
+    uint64_t key_off = 0, sconst = 0, T = 0, cmask = 0;
+    uint32_t m = 0, aux0 = 0, aux1 = 0;
+
+    for (;;) {
+        if (node < 0) {
+            if (!helping) {
+                uint32_t c = 0;
+                if (lane == 0) c = atomicAdd(&a.ctl[0], 1u);
+                c = uni(c);
+                if (c < a.n_nodes) node = (int)c;
+                else helping = true;
+            }
+            if (helping) {
+                int found = -1;
+                for (uint32_t k = 0; k < a.n_waves && found < 0; k += 64) {
+                    uint32_t idx = scan + k + lane;
+                    while (idx >= a.n_waves) idx -= a.n_waves;
+                    int cand = -1;
+                    if (k + lane < a.n_waves) cand = __hip_atomic_load(&a.running[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const bool ok = cand >= 0 && has_work(&a.state[cand], units_per_chunk, scale);
+                    const unsigned long long msk = __ballot(ok);
+                    if (msk) {
+                        const int l = __ffsll((long long)msk) - 1;
+                        found = (int)uni((uint32_t)__shfl(cand, l));
+                        scan = uni(__shfl(idx, l));
+                    }
+                }
+                if (found < 0) break;
+                node = found;
+            }
+            node = (int)uni((uint32_t)node);
+            if (lane == 0) __hip_atomic_store(&a.running[wave], node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const NodeDesc d = a.nodes[node];
+            key_off = uni64(d.key_off);
+            m = uni(d.m);
+            if (KIND == kHigher) {
+                sconst = c_start_seed[uni(d.depth)];
+                aux0 = higher_split(SplitParams{a.leaf, 0, a.upper}, m);           // split
+                aux1 = (uint32_t)(((uint64_t)aux0 * 65536 + m - 1) / m);            // H
+            } else if (KIND == kUpper || KIND == kLower) {
+                sconst = c_start_seed[KIND == kUpper ? kSeedUpper : kSeedLower];
+                const uint32_t su = a.split_unit;
+                const uint32_t fan = (m + su - 1) / su;
+                T = 0;
+                for (uint32_t p = 0; p + 1 < fan; p++) T |= (uint64_t)su << (p * a.field_bits);
+                const uint32_t w = (fan - 1) * a.field_bits;
+                cmask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+            } else if (KIND == kLeafRot) {
+                uint32_t odd = 0;
+                for (uint32_t i = 0; i < m; i++) odd += (uint32_t)(keys[key_off + i] & 1);
+                aux0 = m - odd;  // nl
+                aux1 = odd;      // nr
+            } else {
+                uint32_t mid = m;
+                if (m >= 14) {
+                    // ceil(3 sqrt(m)) for m in [14, 24)
+                    mid = 1;
+                    while (mid * mid < 9 * m) mid++;
+                    if (mid > m) mid = m;
+                }
+                aux0 = mid;
+            }
+        }
+        unsigned long long chunk = 0, res = 0;
+        if (lane == 0) {
+            chunk = atomicAdd(&a.state[node].next_chunk, 1ull);
+            res = ld_relaxed(&a.state[node].result);
+        }
+        chunk = uni64(chunk);
+        res = uni64(res);
+        const uint64_t unit0 = chunk * units_per_chunk;
+        if (unit0 * scale >= res || unit0 * scale >= kSeedLimit) {
+            if (unit0 * scale >= kSeedLimit && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)kSeedLimit);
+            node = -1;
+            if (lane == 0) __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        const uint64_t *kp = keys + key_off;
+        uint64_t hit = kNone;
+        for (uint32_t g = 0; g < a.groups; g++) {
+            const uint64_t u0 = unit0 + (uint64_t)g * U * 64;
+            if (g > 0) {
+                unsigned long long r = 0;
+                if (lane == 0) r = ld_relaxed(&a.state[node].result);
+                r = uni64(r);
+                if (u0 * scale >= r) break;
+            }
+            if (KIND == kHigher) hit = group_higher<U>(kp, m, sconst, aux0, aux1, u0);
+            else if (KIND == kUpper || KIND == kLower)
+                hit = group_aggr<U, WIDE>(kp, m, sconst, a.field_bits, a.magic, T, cmask, u0);
+            else if (KIND == kLeafRot) hit = group_leaf_rot<U>(kp, m, aux0, aux1, u0);
+            else hit = group_leaf_brute<U>(kp, m, aux0, u0);
+            if (hit != kNone) break;
+        }
+        if (hit != kNone && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)hit);
+    }
+    if (lane == 0) __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_state_init(NodeState *st, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        st[i].result = kNone;
+        st[i].next_chunk = 0;
+    }
+}
+
+// Leaves: copy each node's final seed to its slot.
+__global__ __launch_bounds__(256) void k_finalize(const NodeDesc *__restrict__ nodes, const NodeState *__restrict__ st,
+                                                  uint32_t n, uint64_t *__restrict__ seeds, uint32_t *__restrict__ err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t x = st[i].result;
+    seeds[nodes[i].slot] = x;
+    if (x >= kSeedLimit) atomicOr(err, 2u);
+}
+
+// Split nodes: record the seed and partition the node's keys in place by child
+// (stable), one wavefront per node, keys staged in LDS.
+__global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
+                                              const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
+                                              uint32_t *__restrict__ err, SplitParams P, uint32_t kind) {
+    extern __shared__ uint64_t s_keys[];
+    const uint32_t node = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const NodeDesc d = nodes[node];
+    const uint64_t x = st[node].result;
+    if (lane == 0) seeds[d.slot] = x;
+    if (x >= kSeedLimit) {
+        if (lane == 0) atomicOr(err, 2u);
+        return;
+    }
+    const uint32_t m = d.m;
+    uint64_t *kp = keys + d.key_off;
+    for (uint32_t i = lane; i < m; i += 64) s_keys[i] = kp[i];
+    __syncthreads();
+    uint64_t seed_abs;
+    uint32_t unit, fan;
+    if (kind == kHigher) {
+        seed_abs = c_start_seed[d.depth] + x;
+        unit = higher_split(P, m);
+        fan = 2;
+    } else {
+        seed_abs = c_start_seed[kind == kUpper ? kSeedUpper : kSeedLower] + x;
+        unit = kind == kUpper ? P.lower : P.leaf;
+        fan = (m + unit - 1) / unit;
+    }
+    uint32_t base[kMaxFanout];
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxFanout; c++) base[c] = c * unit;
+    const unsigned long long lt = (1ull << lane) - 1;
+    for (uint32_t j0 = 0; j0 < m; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool valid = j < m;
+        const uint64_t key = valid ? s_keys[j] : 0;
+        const uint32_t r = remap16(key + seed_abs, m);
+        const uint32_t cid = kind == kHigher ? (r >= unit ? 1u : 0u) : r / unit;
+        uint32_t pos = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kMaxFanout; c++) {
+            if (c < fan) {
+                const unsigned long long msk = __ballot(valid && cid == c);
+                if (valid && cid == c) pos = base[c] + (uint32_t)__popcll(msk & lt);
+                base[c] += (uint32_t)__popcll(msk);
+            }
+        }
+        if (valid) kp[pos] = key;
+    }
+}
+
+// ------------------------------------------------------------------ encoding
+
+struct EncodeArgs {
+    const uint64_t *acc;        // bucket key offsets, nb+1
+    const uint64_t *node_base;  // bucket slot offsets, nb+1
+    const uint64_t *seeds;
+    const uint16_t *node_m;
+    const uint32_t *memo;
+    uint64_t nb;
+    SplitParams P;
+    int rotate;
+};
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += bcast64_xor(v, o);
+    return v;
+}
+
+// Per bucket (one wavefront): bits of the bucket's descriptor = fixed bits of the
+// whole subtree (memo) + unary bits of every node; algorithmic evaluations per kind.
+__global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__restrict__ bits,
+                                                     unsigned long long *__restrict__ evals) {
+    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = lane_id();
+    if (b >= a.nb) return;
+    const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
+    if (m < 2) {
+        if (lane == 0) bits[b] = 0;
+        return;
+    }
+    const uint64_t base = a.node_base[b];
+    const uint32_t nn = (a.memo[m] >> 16) & 0x7FF;
+    uint64_t un = 0;
+    uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0};
+    for (uint32_t j = lane; j < nn; j += 64) {
+        const uint32_t s = a.node_m[base + j];
+        const uint32_t g = a.memo[s] >> 27;
+        const uint64_t x = a.seeds[base + j];
+        un += (x >> g) + 1;
+        const uint32_t kind = classify(a.P, s, a.rotate != 0);
+        const uint64_t e = kind == kLeafRot ? (x / s + 1) * s : (x + 1) * s;
+#pragma unroll
+        for (int k = 0; k < kNumKinds; k++) ev[k] += (uint32_t)k == kind ? e : 0;
+    }
+    un = wave_sum64(un);
+#pragma unroll
+    for (int k = 0; k < kNumKinds; k++) ev[k] = wave_sum64(ev[k]);
+    if (lane == 0) {
+        bits[b] = (a.memo[m] & 0xFFFF) + un;
+#pragma unroll
+        for (int k = 0; k < kNumKinds; k++)
+            if (ev[k]) atomicAdd(&evals[k], (unsigned long long)ev[k]);
+    }
+}
+
+__device__ __forceinline__ void or_bits(unsigned long long *w, uint64_t pos, uint64_t val, uint32_t width) {
+    if (width == 0 || val == 0) return;
+    const uint64_t wi = pos >> 6, sh = pos & 63;
+    atomicOr(&w[wi], (unsigned long long)(val << sh));
+    if (sh + width > 64) atomicOr(&w[wi + 1], (unsigned long long)(val >> (64 - sh)));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_excl_scan(T v, T &total) {
+    T inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T t = (T)bcast64_up(inc, o);
+        if (lane_id() >= (uint32_t)o) inc += t;
+    }
+    total = (T)bcast64(inc, 63);
+    return inc - v;
+}
+
+// Per bucket (one wavefront): write fixed parts in DFS order, then the unary
+// parts (RiceBitVector.hpp:73-106, SIMDRecSplit.hpp:707-714).
+__global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t *__restrict__ pos,
+                                                    unsigned long long *__restrict__ words) {
+    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = lane_id();
+    if (b >= a.nb) return;
+    const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
+    if (m < 2) return;
+    const uint64_t base = a.node_base[b];
+    const uint32_t nn = (a.memo[m] >> 16) & 0x7FF;
+    const uint64_t p0 = pos[b];
+    const uint64_t fixed_total = a.memo[m] & 0xFFFF;
+    uint64_t fpre = 0, upre = 0;
+    for (uint32_t j0 = 0; j0 < nn; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool valid = j < nn;
+        uint32_t g = 0;
+        uint64_t x = 0;
+        if (valid) {
+            g = a.memo[a.node_m[base + j]] >> 27;
+            x = a.seeds[base + j];
+        }
+        uint64_t ftot, utot;
+        const uint64_t fo = wave_excl_scan<uint64_t>(g, ftot);
+        const uint64_t ul = valid ? (x >> g) + 1 : 0;
+        const uint64_t uo = wave_excl_scan<uint64_t>(ul, utot);
+        if (valid) {
+            or_bits(words, p0 + fpre + fo, x & ((1ull << g) - 1), g);
+            const uint64_t up = p0 + fixed_total + upre + uo + (x >> g);
+            atomicOr(&words[up >> 6], 1ull << (up & 63));
+        }
+        fpre += ftot;
+        upre += utot;
+    }
+}
+
+__global__ void k_rice_sentinel(unsigned long long *words, const uint64_t *pos, uint64_t nb) {
+    const uint64_t p = pos[nb];
+    atomicOr(&words[p >> 6], 1ull << (p & 63));
+}
+
+// ------------------------------------------------------------------ Elias-Fano
+
+struct EfScalars {
+    uint64_t bpk;
+    long long min_delta;
+    long long min_diff;
+    uint64_t total_bits;
+};
+
+// One workgroup: bits_per_key_fixed_point (IEEE double, DoubleEF.hpp:144) and the
+// two minima (DoubleEF.hpp:146-179).
+__global__ __launch_bounds__(1024) void k_ef_scalars(const uint64_t *__restrict__ cum, const uint64_t *__restrict__ pos,
+                                                     uint64_t nb, EfScalars *out) {
+    __shared__ uint64_t s_bpk;
+    __shared__ long long s_a[1024], s_b[1024];
+    if (threadIdx.x == 0) s_bpk = (uint64_t)((double)(1ull << 20) * ((double)pos[nb] / (double)cum[nb]));
+    __syncthreads();
+    const uint64_t bpk = s_bpk;
+    long long md = 0x3FFFFFFFFFFFFFFFll, mf = 0x3FFFFFFFFFFFFFFFll;
+    for (uint64_t i = 1 + threadIdx.x; i <= nb; i += blockDim.x) {
+        const long long d = (long long)(cum[i] - cum[i - 1]);
+        md = d < md ? d : md;
+        const long long bb = (long long)pos[i] - (long long)(bpk * cum[i] >> 20);
+        const long long bp = (long long)pos[i - 1] - (long long)(bpk * cum[i - 1] >> 20);
+        mf = (bb - bp) < mf ? (bb - bp) : mf;
+    }
+    s_a[threadIdx.x] = md;
+    s_b[threadIdx.x] = mf;
+    __syncthreads();
+    for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_a[threadIdx.x] = s_a[threadIdx.x + o] < s_a[threadIdx.x] ? s_a[threadIdx.x + o] : s_a[threadIdx.x];
+            s_b[threadIdx.x] = s_b[threadIdx.x + o] < s_b[threadIdx.x] ? s_b[threadIdx.x + o] : s_b[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out->bpk = bpk;
+        out->min_delta = s_a[0];
+        out->min_diff = s_b[0];
+        out->total_bits = pos[nb];
+    }
+}
+
+struct EfFill {
+    const uint64_t *cum, *pos;
+    uint64_t nb, bpk, lc, lp;
+    long long min_delta, min_diff;
+    unsigned long long *lower, *upper_c, *upper_p;
+    uint8_t *jump;  // bytes
+    uint64_t jump_bytes;
+    uint32_t *err;
+};
+
+// DoubleEF.hpp:204-281: lower bits of both sequences interleaved per bucket, one
+// upper bit per bucket in each upper-bits array.
+__global__ __launch_bounds__(256) void k_ef_fill(EfFill f) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > f.nb) return;
+    const uint64_t cdelta = i * (uint64_t)f.min_delta, bdelta = i * (uint64_t)f.min_diff;
+    const uint64_t cv = f.cum[i] - cdelta;
+    const uint64_t pval = (uint64_t)((long long)f.pos[i] - (long long)(f.bpk * f.cum[i] >> 20)) - bdelta;
+    const uint64_t start = i * (f.lc + f.lp);
+    if (f.lc) or_bits(f.lower, start, cv & ((1ull << f.lc) - 1), (uint32_t)f.lc);
+    if (f.lp) or_bits(f.lower, start + f.lc, pval & ((1ull << f.lp) - 1), (uint32_t)f.lp);
+    const uint64_t uc = (cv >> f.lc) + i, up = (pval >> f.lp) + i;
+    atomicOr(&f.upper_c[uc >> 6], 1ull << (uc & 63));
+    atomicOr(&f.upper_p[up >> 6], 1ull << (up & 63));
+    // jump table (DoubleEF.hpp:283-349): every q-th element records the position
+    // of its upper bit, as a 64-bit head per super block and 16-bit offsets.
+    if ((i & kEfQMaskDev) == 0) {
+        const uint64_t sb = i >> 14;
+        const uint64_t idx = sb * (kEfSuperSizeDev * 2);
+        const uint64_t head = i & ~(uint64_t)((1 << 14) - 1);
+        const uint64_t hc = f.cum[head] - head * (uint64_t)f.min_delta;
+        const uint64_t hp = (uint64_t)((long long)f.pos[head] - (long long)(f.bpk * f.cum[head] >> 20)) -
+                            head * (uint64_t)f.min_diff;
+        const uint64_t head_c = (hc >> f.lc) + head, head_p = (hp >> f.lp) + head;
+        if (i == head) {
+            if ((idx + 1) * 8 + 8 <= f.jump_bytes) {
+                *(uint64_t *)(f.jump + idx * 8) = head_c;
+                *(uint64_t *)(f.jump + idx * 8 + 8) = head_p;
+            }
+        }
+        const uint64_t oc = uc - head_c, op = up - head_p;
+        if (oc >= 65536 || op >= 65536) atomicOr(f.err, 4u);
+        const uint64_t slot = (idx + 2) * 8 + 4 * ((i & ((1 << 14) - 1)) >> 8);
+        if (slot + 4 <= f.jump_bytes) {
+            *(uint16_t *)(f.jump + slot) = (uint16_t)oc;
+            *(uint16_t *)(f.jump + slot + 2) = (uint16_t)op;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ queries
+
+struct QueryArgs {
+    const uint64_t *rice;
+    const uint64_t *lower, *upper_c, *upper_p, *jump;
+    const uint32_t *memo;
+    uint64_t nbuckets, bpk, lc, lp;
+    long long min_delta, min_diff;
+    SplitParams P;
+    int rotate;
+};
+
+__device__ __forceinline__ uint64_t ld_bytes64(const uint64_t *base, uint64_t byte) {
+    const uint64_t w = byte >> 3, sh = (byte & 7) * 8;
+    const uint64_t lo = base[w];
+    return sh ? (lo >> sh) | (base[w + 1] << (64 - sh)) : lo;
+}
+
+__device__ __forceinline__ uint64_t select64(uint64_t x, uint64_t k) {
+    for (uint64_t i = 0; i < k; i++) x &= x - 1;
+    return (uint64_t)__ffsll((long long)x) - 1;
+}
+
+// Batched operator() (AbstractParallelRecSplit.hpp:409-466): one thread per key.
+__global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__restrict__ keys, uint64_t n,
+                                               uint64_t *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t first = keys[2 * t], second = keys[2 * t + 1];
+    const uint64_t i = __umul64hi(first, q.nbuckets);
+    // DoubleEF::get (DoubleEF.hpp:367-402)
+    const uint64_t lc = q.lc, lp = q.lp;
+    const uint64_t mask_c = (1ull << lc) - 1, mask_p = (1ull << lp) - 1;
+    const uint64_t pos_lower = i * (lc + lp);
+    uint64_t lower = ld_bytes64(q.lower, pos_lower / 8) >> (pos_lower % 8);
+    const uint64_t jsq = (i >> 14) * kEfSuperSizeDev * 2;
+    const uint64_t jin = (i & ((1 << 14) - 1)) >> 8;
+    const uint64_t pair = ld_bytes64(q.jump, (jsq + 2) * 8 + 4 * jin);
+    const uint64_t jump_c = q.jump[jsq] + (pair & 0xFFFF);
+    const uint64_t jump_p = q.jump[jsq + 1] + ((pair >> 16) & 0xFFFF);
+    uint64_t wc = jump_c / 64, wp = jump_p / 64;
+    uint64_t win_c = q.upper_c[wc] & (~0ull << (jump_c % 64));
+    uint64_t win_p = q.upper_p[wp] & (~0ull << (jump_p % 64));
+    uint64_t dc = i & 255, dp = i & 255;
+    for (uint64_t c; (c = (uint64_t)__popcll(win_c)) <= dc; dc -= c) win_c = q.upper_c[++wc];
+    for (uint64_t c; (c = (uint64_t)__popcll(win_p)) <= dp; dp -= c) win_p = q.upper_p[++wp];
+    const uint64_t sel_c = select64(win_c, dc);
+    const uint64_t cum_delta = i * (uint64_t)q.min_delta;
+    uint64_t cum_keys = ((wc * 64 + sel_c - i) << lc | (lower & mask_c)) + cum_delta;
+    lower >>= lc;
+    const uint64_t bit_pos = ((wp * 64 + select64(win_p, dp) - i) << lp | (lower & mask_p)) + i * (uint64_t)q.min_diff +
+                             (uint64_t)(long long)(q.bpk * cum_keys >> 20);
+    win_c &= (~0ull << sel_c) << 1;
+    while (win_c == 0) win_c = q.upper_c[++wc];
+    lower >>= lp;
+    const uint64_t cum_next = ((wc * 64 + (uint64_t)__ffsll((long long)win_c) - 1 - i - 1) << lc | (lower & mask_c)) +
+                              cum_delta + (uint64_t)q.min_delta;
+    uint64_t m = cum_next - cum_keys;
+    // Golomb-Rice reader (RiceBitVector.hpp:155-218)
+    uint64_t fixed_off = bit_pos;
+    uint64_t up = bit_pos + (q.memo[m] & 0xFFFF);
+    uint64_t uw = up / 64;
+    uint64_t window = q.rice[uw++] >> (up & 63);
+    int valid = 64 - (int)(up & 63);
+    auto read_next = [&](uint32_t g) -> uint64_t {
+        uint64_t result = 0;
+        if (window == 0) {
+            result += (uint64_t)valid;
+            window = q.rice[uw++];
+            valid = 64;
+            while (window == 0) {
+                result += 64;
+                window = q.rice[uw++];
+            }
+        }
+        const uint64_t p = (uint64_t)__ffsll((long long)window) - 1;
+        window >>= p;
+        window >>= 1;
+        valid -= (int)p + 1;
+        result = (result + p) << g;
+        result |= (ld_bytes64(q.rice, fixed_off / 8) >> (fixed_off % 8)) & ((1ull << g) - 1);
+        fixed_off += g;
+        return result;
+    };
+    auto skip = [&](uint64_t nodes, uint64_t flen) {
+        uint64_t missing = nodes, c;
+        while ((c = (uint64_t)__popcll(window)) < missing) {
+            window = q.rice[uw++];
+            missing -= c;
+            valid = 64;
+        }
+        c = select64(window, missing - 1);
+        window >>= c;
+        window >>= 1;
+        valid -= (int)c + 1;
+        fixed_off += flen;
+    };
+    const SplitParams P = q.P;
+    int level = 0;
+    while (m > P.upper) {
+        const uint64_t d = read_next(q.memo[m] >> 27);
+        const uint32_t hmod = remap16(second + d + c_start_seed[level], (uint32_t)m);
+        const uint32_t split = higher_split(P, (uint32_t)m);
+        if (hmod < split) {
+            m = split;
+        } else {
+            skip((q.memo[split] >> 16) & 0x7FF, q.memo[split] & 0xFFFF);
+            m -= split;
+            cum_keys += split;
+        }
+        level++;
+    }
+    if (m > P.lower) {
+        const uint64_t d = read_next(q.memo[m] >> 27);
+        const uint32_t hmod = remap16(second + d + c_start_seed[kSeedUpper], (uint32_t)m);
+        const uint64_t part = (uint16_t)hmod / P.lower;
+        m = min((uint64_t)P.lower, m - part * P.lower);
+        cum_keys += P.lower * part;
+        if (part) skip(((q.memo[P.lower] >> 16) & 0x7FF) * part, (q.memo[P.lower] & 0xFFFF) * part);
+    }
+    if (m > P.leaf) {
+        const uint64_t d = read_next(q.memo[m] >> 27);
+        const uint32_t hmod = remap16(second + d + c_start_seed[kSeedLower], (uint32_t)m);
+        const uint64_t part = (uint16_t)hmod / P.leaf;
+        m = min((uint64_t)P.leaf, m - part * P.leaf);
+        cum_keys += (uint64_t)P.leaf * part;
+        if (part) skip(part, (q.memo[P.leaf] & 0xFFFF) * part);
+    }
+    const uint64_t b = read_next(q.memo[m] >> 27);
+    const uint64_t x = second + b + c_start_seed[kSeedLeaf];
+    uint64_t v;
+    if (q.rotate && m == P.leaf) {
+        const uint64_t rot = b % P.leaf;
+        v = remap16(x - rot, P.leaf);
+        if (second & 1) v = (v + rot) % P.leaf;
+    } else {
+        v = remap16(x, (uint32_t)m);
+    }
+    out[t] = cum_keys + v;
+}
+
+}  // namespace rsa

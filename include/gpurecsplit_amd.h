@@ -1,0 +1,131 @@
+/*
+ * gpurecsplit_amd.h -- C ABI of the MI355X (gfx950) RecSplit construction engine.
+ *
+ * A RecSplit minimal perfect hash function (MPHF) maps n distinct 128-bit key
+ * hashes {first, second} onto [0, n).  This library builds it on one AMD GPU and
+ * returns a handle whose serialized bytes follow the reference's stream layout
+ * (leaf size, bucket size, key count, Golomb-Rice descriptor vector, double
+ * Elias-Fano structure), and which answers queries on the host or, batched, on
+ * the GPU.
+ *
+ * Reference interfaces these entry points replace (paths relative to the
+ * ByteHamster/GpuRecSplit repository root):
+ *   rs_build / rs_build_device
+ *       GPURecSplit(vector<hash128_t>& keys, size_t bucket_size, int numThreads)
+ *       include/function/GPURecSplit.cuh:484-489 (construction: hash_gen, :637-815)
+ *   rs_mphf_serialize / rs_mphf_serialized_size
+ *       operator<<  include/function/GPURecSplit.cuh:817-825
+ *   rs_mphf_deserialize
+ *       operator>>  include/function/GPURecSplit.cuh:827-841
+ *   rs_mphf_query
+ *       size_t operator()(const hash128_t&)
+ *       include/function/AbstractParallelRecSplit.hpp:409-466
+ *   rs_mphf_bits
+ *       getBits()   include/function/AbstractParallelRecSplit.hpp:476-478
+ *       (without the 8*sizeof(object) term, which the C++ wrapper adds)
+ *   rs_query_device
+ *       batched operator() on the GPU (no reference counterpart; the reference
+ *       queries one key at a time on the host)
+ *
+ * Errors are returned as status codes (the reference prints and calls exit() on
+ * CUDA errors, GPURecSplit.cuh:73-80, loops forever on duplicate keys, :436,
+ * and throws std::out_of_range for a bucket of 3000 or more keys, :682).
+ * rs_last_error() gives a message for the calling thread's last failure.
+ *
+ * Keys are passed as n pairs of uint64 (first, second), little-endian, i.e. the
+ * memory layout of the reference's hash128_t array.  The caller's keys are
+ * never modified (the reference sorts the caller's vector in place,
+ * AbstractParallelRecSplit.hpp:344).
+ *
+ * Thread safety: a context serialises builds issued through it; use one context
+ * per host thread for concurrent builds on one device.
+ */
+#ifndef GPURECSPLIT_AMD_H
+#define GPURECSPLIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rs_ctx rs_ctx;
+typedef struct rs_mphf rs_mphf;
+
+enum rs_status {
+    RS_OK = 0,
+    RS_ERR_ARG = 1,               /* invalid argument (n == 0, leaf size outside [2, 24], ...) */
+    RS_ERR_HIP = 2,               /* HIP runtime error */
+    RS_ERR_BUCKET_TOO_LARGE = 3,  /* a bucket holds 3000 or more keys */
+    RS_ERR_DUPLICATE_KEYS = 4,    /* two keys in one bucket share `second` */
+    RS_ERR_SEED_OVERFLOW = 5,     /* a node search passed the 2^44 seed limit */
+    RS_ERR_EF_OVERFLOW = 6,       /* Elias-Fano jump offset >= 2^16 (reference aborts) */
+    RS_ERR_NOMEM = 7,             /* host or device allocation failed */
+    RS_ERR_FORMAT = 8,            /* malformed serialized stream or leaf-size mismatch */
+    RS_ERR_NO_DEVICE = 9          /* handle has no device copy (deserialized) */
+};
+
+/* Per-build measurements (HIP events on the build stream). */
+typedef struct rs_build_stats {
+    double total_ms;          /* host wall clock of the whole build call */
+    double partition_ms;      /* histogram + scans + scatter + per-bucket sort */
+    double search_ms[8];      /* search kernels: [0] fanout-2 levels, [1] upper
+                                 aggregation, [2] lower aggregation, [3] rotation
+                                 leaves, [4] brute-force leaves */
+    double split_ms;          /* partition of keys between tree levels */
+    double encode_ms;         /* Golomb-Rice + Elias-Fano */
+    uint64_t evals[8];        /* algorithmic key-seed evaluations per kind (same index) */
+    uint64_t nodes[8];        /* nodes per kind */
+    uint64_t num_buckets;
+    uint64_t num_nodes;
+    uint64_t rice_bits;
+    uint32_t max_bucket_size;
+    uint32_t search_waves;    /* persistent waves per search launch */
+} rs_build_stats;
+
+/* Context bound to one GPU (device ordinal as seen by HIP). */
+int rs_ctx_create(int device, rs_ctx **out);
+void rs_ctx_destroy(rs_ctx *ctx);
+
+/* Construction from host keys (n pairs first,second).  leaf_size in [2, 24],
+ * bucket_size >= 1, rotate != 0 selects rotation fitting for full leaves
+ * (USE_BIJECTIONS_ROTATE, GPURecSplit.cuh:409). */
+int rs_build(rs_ctx *ctx, const uint64_t *keys, size_t n, uint32_t leaf_size, size_t bucket_size,
+             int rotate, rs_mphf **out);
+
+/* Same, with the keys already resident in device memory of ctx's GPU. */
+int rs_build_device(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf_size,
+                    size_t bucket_size, int rotate, rs_mphf **out);
+
+/* Serialization in the reference's stream layout. */
+size_t rs_mphf_serialized_size(const rs_mphf *h);
+int rs_mphf_serialize(const rs_mphf *h, uint8_t *buf, size_t cap);
+int rs_mphf_deserialize(const uint8_t *buf, size_t len, uint32_t leaf_size, int rotate, rs_mphf **out);
+
+/* Host query of one key: its index in [0, n). */
+uint64_t rs_mphf_query(const rs_mphf *h, uint64_t first, uint64_t second);
+/* Host query of n keys. */
+void rs_mphf_query_batch(const rs_mphf *h, const uint64_t *keys, size_t n, uint64_t *out);
+/* Batched query on the GPU: d_keys (n pairs) and d_out (n uint64) in device memory
+ * of ctx's GPU; h must come from a build on that context. */
+int rs_query_device(rs_ctx *ctx, const rs_mphf *h, const uint64_t *d_keys, size_t n, uint64_t *d_out);
+
+/* Space of the structure in bits (Elias-Fano + descriptors). */
+uint64_t rs_mphf_bits(const rs_mphf *h);
+uint64_t rs_mphf_num_keys(const rs_mphf *h);
+int rs_mphf_stats(const rs_mphf *h, rs_build_stats *out);
+void rs_mphf_free(rs_mphf *h);
+
+/* Golomb-Rice memo for a leaf size: golomb parameter << 27 | subtree nodes << 16 |
+ * subtree fixed bits, for bucket sizes 0..2999 (AbstractParallelRecSplit.hpp:191-227,
+ * golombRiceMemo.hpp). */
+int rs_golomb_memo(uint32_t leaf_size, uint32_t *out3000);
+
+const char *rs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPURECSPLIT_AMD_H */

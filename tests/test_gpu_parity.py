@@ -1,0 +1,129 @@
+"""GPU parity: the HIP build's serialized MPHF is byte-identical to the oracle's
+on the same keys, every key maps to a distinct index in [0, n), and the GPU
+query kernel agrees with the host query.  Runs through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleMPHF, splitmix_keys
+
+pytestmark = pytest.mark.gpu
+
+gra = pytest.importorskip("gpurecsplit_amd")
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _parity(keys, leaf, bucket, rotate=True, threads=8):
+    _log(f"parity leaf={leaf} bucket={bucket} n={keys.shape[0]} rotate={rotate}")
+    rs = gra.GPURecSplit(keys, bucket, leaf_size=leaf, rotate=rotate)
+    ref = OracleMPHF.build(keys, leaf, bucket, rotate=rotate, threads=threads)
+    got, exp = rs.serialize(), ref.serialize()
+    if got != exp:
+        # locate the first differing seed for a readable failure
+        g = OracleMPHF.load(got, leaf, rotate)
+        gs, es = g.seeds(), ref.seeds()
+        k = next((i for i in range(min(len(gs), len(es))) if gs[i] != es[i]), None)
+        pytest.fail(f"bytes differ (len {len(got)} vs {len(exp)}); first differing seed index {k}: "
+                    f"{gs[k] if k is not None else None} vs {es[k] if k is not None else None}")
+    q = rs.query(keys)
+    assert np.array_equal(np.sort(q), np.arange(keys.shape[0], dtype=np.uint64))
+    return rs
+
+
+CASES = [
+    (2, 7, 2000), (3, 4, 1000), (4, 25, 3000), (5, 10, 5000), (6, 1009, 6000), (7, 100, 10000),
+    (8, 100, 50000), (8, 2, 300), (8, 2000, 20000), (9, 231, 8000), (10, 483, 10000), (11, 75, 5000),
+    (12, 100, 30000), (13, 1300, 8000), (14, 231, 4000), (16, 2000, 10000), (16, 100, 5000),
+    (18, 50, 3000), (20, 100, 500), (24, 10, 100), (8, 5, 1), (8, 100, 10), (16, 2000, 123),
+]
+
+
+@pytest.mark.parametrize("leaf,bucket,n", CASES)
+def test_bytes_match_oracle(torch_cuda, leaf, bucket, n):
+    keys = splitmix_keys(leaf * 7919 + bucket + n, n)
+    _parity(keys, leaf, bucket)
+
+
+@pytest.mark.parametrize("leaf,bucket,n", [(2, 7, 2000), (5, 10, 5000), (8, 100, 20000), (11, 75, 3000)])
+def test_bytes_match_oracle_bruteforce_leaves(torch_cuda, leaf, bucket, n):
+    keys = splitmix_keys(leaf + 5, n)
+    _parity(keys, leaf, bucket, rotate=False)
+
+
+def test_reference_cpu_config_1m_l8_b100(torch_cuda):
+    """n=1,000,000 splitmix64(42) keys, l=8, b=100, rotation: GPU bytes == oracle bytes."""
+    keys = splitmix_keys(42, 1_000_000)
+    _parity(keys, 8, 100)
+
+
+def test_device_query_matches_host(torch_cuda):
+    torch = torch_cuda
+    keys = splitmix_keys(9, 200_000)
+    rs = gra.GPURecSplit(keys, 500, leaf_size=10)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    out = rs.query_device(d).cpu().numpy().view(np.uint64)
+    assert np.array_equal(out, rs.query(keys))
+    assert np.array_equal(np.sort(out), np.arange(200_000, dtype=np.uint64))
+
+
+def test_device_resident_keys_build(torch_cuda):
+    torch = torch_cuda
+    keys = splitmix_keys(10, 40_000)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    a = gra.GPURecSplit(d, 300, leaf_size=9).serialize()
+    b = gra.GPURecSplit(keys, 300, leaf_size=9).serialize()
+    assert a == b
+
+
+def test_repeated_builds_reuse_context(torch_cuda):
+    """Buffers are reused across builds; a bigger then a smaller build must both
+    match the oracle."""
+    for n in (30_000, 2_000, 30_000):
+        keys = splitmix_keys(n, n)
+        _parity(keys, 8, 150)
+
+
+def test_duplicate_keys_rejected(torch_cuda):
+    keys = splitmix_keys(11, 1000)
+    keys[10] = keys[20]
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit(keys, 50, leaf_size=8)
+    assert e.value.code == 4
+
+
+def test_equal_second_in_bucket_rejected(torch_cuda):
+    keys = splitmix_keys(12, 1000)
+    keys[30, 1] = keys[31, 1]
+    keys[30, 0] = keys[31, 0] ^ 1  # same bucket (differ in the lowest bit of first)
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit(keys, 50, leaf_size=8)
+    assert e.value.code == 4
+
+
+def test_oversized_bucket_rejected(torch_cuda):
+    keys = splitmix_keys(13, 3500)
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit(keys, 5000, leaf_size=8)
+    assert e.value.code == 3
+
+
+def test_stats_and_evals(torch_cuda):
+    keys = splitmix_keys(14, 50_000)
+    rs = gra.GPURecSplit(keys, 2000, leaf_size=12)
+    st = rs.stats()
+    ref = OracleMPHF.build(keys, 12, 2000, threads=8)
+    assert sum(st["evals"].values()) == ref.evals
+    assert st["num_nodes"] == ref.nseeds
