@@ -62,6 +62,14 @@ class BuildStats(C.Structure):
 
 
 def _load():
+    # One HIP runtime per process: torch ships its own libamdhip64 (same SONAME,
+    # loaded under another file name by torch's libraries).  Importing torch first
+    # makes this library bind to that already-loaded runtime instead of loading a
+    # second one from /opt/rocm.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built: run `make -C gpurecsplit_amd/csrc` (hipcc, gfx950)")
     lib = C.CDLL(str(LIB_PATH))
@@ -136,9 +144,9 @@ class Context:
             cls._default[device] = cls(device)
         return cls._default[device]
 
-    def close(self):
+    def close(self, _destroy=lib.rs_ctx_destroy):
         if getattr(self, "h", None):
-            lib.rs_ctx_destroy(self.h)
+            _destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -231,7 +239,7 @@ class GPURecSplit:
         _check(lib.rs_mphf_stats(self.h, C.byref(s)))
         return s.as_dict()
 
-    def __del__(self):
+    def __del__(self, _free=lib.rs_mphf_free):
         if getattr(self, "h", None):
-            lib.rs_mphf_free(self.h)
+            _free(self.h)
             self.h = None
