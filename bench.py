@@ -64,6 +64,24 @@ def cpu_baseline(leaf: int, bucket: int, seed: int) -> dict:
                       f"l={leaf}, rotation on, {threads} threads, wall {dt:.1f} s"}
 
 
+def probe(device: int = 0) -> dict:
+    """VALU issue rates (lane-ops/s) of the instructions of the seed-search inner
+    loop, from rs_probe.hip, plus one calibration copy of 256 MiB."""
+    import ctypes as C
+
+    lib = C.CDLL(str(ROOT / "gpurecsplit_amd" / "librs_probe.so"))
+    rates = {}
+    for kind, name in enumerate(["v_xor_b32", "v_mul_lo_u32", "v_lshl_add_u64", "v_mul_u32_u24"]):
+        r, ms = C.c_double(), C.c_double()
+        assert lib.rs_probe_valu(device, kind, 256, C.byref(r), C.byref(ms)) == 0
+        rates[name] = r.value
+    ms = C.c_double()
+    assert lib.rs_probe_copy(device, 256 << 20, C.byref(ms)) == 0
+    base = rates["v_xor_b32"]
+    return {"lane_ops_per_s": rates, "issue_slots": {k: base / v for k, v in rates.items()},
+            "copy_256MiB_ms": ms.value, "copy_GBps": 2 * (256 << 20) / (ms.value / 1e3) / 1e9}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +92,7 @@ def main() -> int:
     ap.add_argument("--bucket", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="also run the VALU issue-rate and copy probes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -143,11 +162,15 @@ def main() -> int:
                      "traffic": None, "evals": evals, "kernel_ms": kms,
                      "ops_per_eval": OPS_PER_EVAL[kind]},
         "seeds_evals_per_s": total_evals / (search_total_ms / 1e3),
+        "exec_ratio": {k: (last["executed_evals"][k] / last["evals"][k]) if last["evals"][k] else None
+                       for k in last["evals"]},
         "bits_per_key": bits_per_key,
         "bijection_verified": ok,
         "phase_ms": {"partition": last["partition_ms"], "search": last["search_ms"], "split": last["split_ms"],
                      "encode": last["encode_ms"], "total": last["total_ms"]},
     }
+    if args.probe:
+        line["probe"] = probe(0)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.leaf, args.bucket, args.seed)
     print(json.dumps(line))

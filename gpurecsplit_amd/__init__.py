@@ -44,7 +44,7 @@ class BuildStats(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double), ("partition_ms", C.c_double), ("search_ms", C.c_double * 8),
         ("split_ms", C.c_double), ("encode_ms", C.c_double), ("evals", C.c_uint64 * 8),
-        ("nodes", C.c_uint64 * 8), ("num_buckets", C.c_uint64), ("num_nodes", C.c_uint64),
+        ("nodes", C.c_uint64 * 8), ("executed_evals", C.c_uint64 * 8), ("num_buckets", C.c_uint64), ("num_nodes", C.c_uint64),
         ("rice_bits", C.c_uint64), ("max_bucket_size", C.c_uint32), ("search_waves", C.c_uint32),
     ]
 
@@ -55,6 +55,7 @@ class BuildStats(C.Structure):
             "search_ms": {k: self.search_ms[i] for i, k in enumerate(KIND_NAMES)},
             "evals": {k: int(self.evals[i]) for i, k in enumerate(KIND_NAMES)},
             "nodes": {k: int(self.nodes[i]) for i, k in enumerate(KIND_NAMES)},
+            "executed_evals": {k: int(self.executed_evals[i]) for i, k in enumerate(KIND_NAMES)},
             "num_buckets": int(self.num_buckets), "num_nodes": int(self.num_nodes),
             "rice_bits": int(self.rice_bits), "max_bucket_size": int(self.max_bucket_size),
             "search_waves": int(self.search_waves),

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -181,14 +182,51 @@ static void census_node(const SplitParams &P, bool rotate, uint32_t s, uint32_t 
 
 using namespace rsa;
 
+// Launch configuration of the seed search, per node kind (index = NodeKind).
+// Defaults are the measured best on MI355X; RS_TUNE overrides them, e.g.
+// RS_TUNE="u=2,lut=1,g1=1,g2=2,g3=4,bpc=8" (u: candidates in flight per lane,
+// lut: LDS increment table for aggregation, gK: 64*u-candidate groups per claimed
+// chunk of kind K, bpc: workgroups of 4 waves per CU).
+struct Tune {
+    int u = 2;
+    int lut = 1;
+    int groups[kNumKinds] = {1, 1, 2, 4, 4};
+    int blocks_per_cu = 8;
+};
+
+static Tune parse_tune(const char *env) {
+    Tune t;
+    if (!env) return t;
+    std::string s(env);
+    size_t p = 0;
+    while (p < s.size()) {
+        size_t q = s.find(',', p);
+        if (q == std::string::npos) q = s.size();
+        const std::string kv = s.substr(p, q - p);
+        const size_t e = kv.find('=');
+        if (e != std::string::npos) {
+            const std::string k = kv.substr(0, e);
+            const int v = std::atoi(kv.c_str() + e + 1);
+            if (k == "u" && (v == 1 || v == 2 || v == 4)) t.u = v;
+            else if (k == "lut") t.lut = v != 0;
+            else if (k == "bpc" && v >= 1 && v <= 32) t.blocks_per_cu = v;
+            else if (k.size() == 2 && k[0] == 'g' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && v >= 1 && v <= 64)
+                t.groups[k[1] - '0'] = v;
+        }
+        p = q + 1;
+    }
+    return t;
+}
+
 struct rs_ctx {
+    Tune tune;
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
-        list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, rice, ef_scalars, ef_lower, ef_uc, ef_up,
-        ef_jump;
+        list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
+        ef_up, ef_jump;
     uint32_t *memo[kMaxLeaf + 1] = {};
     uint32_t *h_pinned = nullptr;  // small pinned staging area
     hipEvent_t ev[128];
@@ -271,21 +309,37 @@ static double ev_ms(rs_ctx *c, int a, int b) {
     return ms;
 }
 
-static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, SearchArgs a, bool wide, uint32_t blocks) {
-    constexpr int U = 2;
+template <int KIND, int U>
+static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
+                        size_t lds) {
     hipStream_t s = c->stream;
+    if (KIND == kUpper || KIND == kLower) {
+        if (wide) k_search<KIND, U, true, false><<<blocks, 256, 0, s>>>(keys, a);
+        else if (lut) k_search<KIND, U, false, true><<<blocks, 256, lds, s>>>(keys, a);
+        else k_search<KIND, U, false, false><<<blocks, 256, 0, s>>>(keys, a);
+    } else {
+        k_search<KIND, U, false, false><<<blocks, 256, 0, s>>>(keys, a);
+    }
+}
+
+template <int U>
+static void launch_u(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
+                     uint32_t blocks, size_t lds) {
     switch (kind) {
-    case kHigher: k_search<kHigher, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
-    case kUpper:
-        if (wide) k_search<kUpper, U, true><<<blocks, 256, 0, s>>>(keys, a);
-        else k_search<kUpper, U, false><<<blocks, 256, 0, s>>>(keys, a);
-        break;
-    case kLower:
-        if (wide) k_search<kLower, U, true><<<blocks, 256, 0, s>>>(keys, a);
-        else k_search<kLower, U, false><<<blocks, 256, 0, s>>>(keys, a);
-        break;
-    case kLeafRot: k_search<kLeafRot, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
-    default: k_search<kLeafBrute, U, false><<<blocks, 256, 0, s>>>(keys, a); break;
+    case kHigher: launch_kind<kHigher, U>(c, keys, a, wide, lut, blocks, lds); break;
+    case kUpper: launch_kind<kUpper, U>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLower: launch_kind<kLower, U>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLeafRot: launch_kind<kLeafRot, U>(c, keys, a, wide, lut, blocks, lds); break;
+    default: launch_kind<kLeafBrute, U>(c, keys, a, wide, lut, blocks, lds); break;
+    }
+}
+
+static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
+                          uint32_t blocks, size_t lds) {
+    switch (c->tune.u) {
+    case 1: launch_u<1>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    case 4: launch_u<4>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    default: launch_u<2>(c, kind, keys, a, wide, lut, blocks, lds); break;
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -351,12 +405,12 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     }
 
     // ---- 2. scatter and per-bucket sort
-    uint64_t *sorted = c->sorted.get<uint64_t>(n);
+    uint64_t *sorted = c->sorted.get<uint64_t>(n + 8);  // +8: look-ahead loads of the search
     uint32_t *cursor = c->cursor.get<uint32_t>(nb);
     HIP_CHECK(hipMemsetAsync(cursor, 0, nb * 4, s));
     k_scatter<<<hist_blocks, 256, 0, s>>>(d_keys, n, nb, acc, cursor, sorted);
     HIP_CHECK(hipGetLastError());
-    k_sort_buckets<<<nb, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err);
+    k_sort_buckets<<<nb, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err, rotate ? leaf : 0);
     HIP_CHECK(hipGetLastError());
 
     // ---- 3. node lists
@@ -380,11 +434,14 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     uint64_t max_list = 1;
     for (int l = 0; l < kNumLists; l++) max_list = std::max(max_list, cen.count[l]);
     NodeState *state = c->state.get<NodeState>(max_list);
-    const uint32_t blocks = (uint32_t)c->num_cus * 8;
+    const Tune &tn = c->tune;
+    const uint32_t blocks = (uint32_t)c->num_cus * (uint32_t)tn.blocks_per_cu;
     const uint32_t n_waves = blocks * 4;
     st.search_waves = n_waves;
     int *running = c->running.get<int>(n_waves);
     uint32_t *ctl = c->ctl.get<uint32_t>(16);
+    unsigned long long *work = c->work.get<unsigned long long>(kNumLists);
+    HIP_CHECK(hipMemsetAsync(work, 0, kNumLists * 8, s));
     const AggrLayout up_l = aggr_layout(P.lower, P.upper / P.lower);
     const AggrLayout lo_l = aggr_layout(P.leaf, P.lower / P.leaf);
     struct PhaseEvents {
@@ -409,27 +466,21 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
         a.n_waves = n_waves;
         a.leaf = P.leaf;
         a.upper = P.upper;
+        a.work = work + l;
+        a.groups = (uint32_t)tn.groups[kind];
         bool wide = false;
-        switch (kind) {
-        case kHigher: a.groups = 1; break;
-        case kUpper:
-            a.groups = 1;
-            a.split_unit = P.lower;
-            a.field_bits = up_l.fb;
-            a.magic = up_l.magic;
-            wide = up_l.wide;
-            break;
-        case kLower:
-            a.groups = 2;
-            a.split_unit = P.leaf;
-            a.field_bits = lo_l.fb;
-            a.magic = lo_l.magic;
-            wide = lo_l.wide;
-            break;
-        default: a.groups = 4; break;
+        size_t lds = 0;
+        if (kind == kUpper || kind == kLower) {
+            const AggrLayout &L = kind == kUpper ? up_l : lo_l;
+            a.split_unit = kind == kUpper ? P.lower : P.leaf;
+            a.field_bits = L.fb;
+            a.magic = L.magic;
+            wide = L.wide;
+            a.lut_stride = ((kind == kUpper ? P.upper : P.lower) + 3) & ~3u;
+            lds = (size_t)4 * a.lut_stride * 4;
         }
         PhaseEvents pe{kind, ev_record(c), -1, -1};
-        launch_search(c, kind, sorted, a, wide, blocks);
+        launch_search(c, kind, sorted, a, wide, tn.lut != 0, blocks, lds);
         pe.search1 = ev_record(c);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);
@@ -455,10 +506,13 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     uint64_t *h64 = (uint64_t *)c->h_pinned;
     HIP_CHECK(hipMemcpyAsync(h64, pos + nb, 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h64 + 1, evals, kNumKinds * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h64 + 16, work, kNumLists * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     const uint64_t total_bits = h64[0];
     for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
+    for (int l = 0; l < kNumLists; l++)
+        st.executed_evals[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += h64[16 + l];
     if (h[16] & 2) throw BuildError(RS_ERR_SEED_OVERFLOW, "a node search passed the 2^44 seed limit");
     const uint64_t B = total_bits + 1;  // with the sentinel
     const uint64_t rice_words = ((B + 7) / 8 + 14) / 8;
@@ -604,6 +658,7 @@ int rs_ctx_create(int device, rs_ctx **out) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount;
+        c->tune = parse_tune(getenv("RS_TUNE"));
         HIP_CHECK(hipStreamCreate(&c->stream));  // blocking: ordered after null-stream work
         HIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * 1024, hipHostMallocDefault));
         for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
@@ -622,7 +677,7 @@ void rs_ctx_destroy(rs_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->keys_in, &c->counts, &c->counts64, &c->cursor, &c->acc, &c->node_cnt, &c->node_base,
                       &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
-                      &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->rice,
+                      &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
                       &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump})
         b->release();
     for (auto &m : c->memo)

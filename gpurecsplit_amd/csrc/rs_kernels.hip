@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_scatter(const uint64_t *__restrict__ ke
 // (canonical order, independent of the scatter's atomics) and detection of equal
 // values, which no seed can ever separate.
 __global__ __launch_bounds__(256) void k_sort_buckets(uint64_t *__restrict__ sorted, const uint64_t *__restrict__ acc,
-                                                      uint32_t *__restrict__ err) {
+                                                      uint32_t *__restrict__ err, uint32_t rot_leaf) {
     __shared__ uint64_t s[4096];
     const uint64_t b = blockIdx.x;
     const uint64_t off = acc[b];
@@ -145,10 +145,19 @@ __global__ __launch_bounds__(256) void k_sort_buckets(uint64_t *__restrict__ sor
             __syncthreads();
         }
     }
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        sorted[off + i] = s[i];
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
         if (i > 0 && s[i] == s[i - 1]) atomicOr(err, 1u);
+    if (m == rot_leaf) {
+        // a bucket that is one rotation leaf: even keys first (see k_split)
+        if (threadIdx.x == 0) {
+            uint32_t o = 0;
+            for (uint32_t par = 0; par < 2; par++)
+                for (uint32_t i = 0; i < m; i++)
+                    if ((s[i] & 1) == par) sorted[off + o++] = s[i];
+        }
+        return;
     }
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) sorted[off + i] = s[i];
 }
 
 // One thread per bucket: walks the bucket's split tree in DFS preorder and emits
@@ -224,7 +233,8 @@ struct SearchArgs {
     const NodeDesc *nodes;
     NodeState *state;
     int *running;
-    uint32_t *ctl;  // [0] fresh-node cursor
+    uint32_t *ctl;               // [0] fresh-node cursor
+    unsigned long long *work;    // executed lane evaluations of this launch (diagnostic)
     uint32_t n_nodes;
     uint32_t n_waves;
     uint32_t groups;      // groups of 64*U candidates per claimed chunk
@@ -233,7 +243,32 @@ struct SearchArgs {
     uint32_t field_bits;  // packed counter field width
     uint32_t magic;       // ceil(2^20 / split_unit)
     uint32_t upper;       // upper aggregation bound (fanout-2 split size)
+    uint32_t lut_stride;  // LDS words per wave for the aggregation increment table
 };
+
+// Streams keys kp[begin, end) through scalar registers; the next four keys are
+// loaded while the current four are hashed.  The key array carries 8 entries of
+// padding, so the look-ahead loads (up to kp[end + 3]) stay inside it.
+template <typename F>
+__device__ __forceinline__ void stream_keys(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end,
+                                            uint64_t sconst, F &&f) {
+    uint32_t i = begin;
+    uint64_t n0 = kp[i], n1 = kp[i + 1], n2 = kp[i + 2], n3 = kp[i + 3];
+    for (; i + 4 <= end; i += 4) {
+        const uint64_t c0 = n0 + sconst, c1 = n1 + sconst, c2 = n2 + sconst, c3 = n3 + sconst;
+        n0 = kp[i + 4];
+        n1 = kp[i + 5];
+        n2 = kp[i + 6];
+        n3 = kp[i + 7];
+        f(c0);
+        f(c1);
+        f(c2);
+        f(c3);
+    }
+    if (i < end) f(n0 + sconst);
+    if (i + 1 < end) f(n1 + sconst);
+    if (i + 2 < end) f(n2 + sconst);
+}
 
 // Fanout-2 node: lowest x with #{remap(k+S[depth]+x, m) < split} == split
 // (SIMDRecSplit.hpp:250-317).  remap < split <=> h16 < ceil(split*2^16/m).
@@ -248,12 +283,10 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
         s[u] = u0 + (uint64_t)(u * 64 + lane);
         cnt[u] = 0;
     }
-#pragma unroll 4
-    for (uint32_t i = 0; i < m; i++) {
-        const uint64_t k = kp[i] + sconst;
+    stream_keys(kp, 0, m, sconst, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) cnt[u] += fold_h16(k + s[u]) < H;
-    }
+    });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(cnt[u] == split);
@@ -266,10 +299,12 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
 // expected counts (SIMDRecSplit.hpp:319-423).  Counts are packed in fields of
 // `fb` bits (2^fb >= SPLIT + 2), the last part uncounted; a carry out of any field
 // forces another count negative, so equality of the counted fields is
-// equality of all counts.
-template <int U, bool WIDE>
+// equality of all counts.  With LUT, the increment 1 << (fb * part) of every
+// remapped value r < m is read from a per-wave LDS table instead of computed.
+template <int U, bool WIDE, bool LUT>
 __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
-                                               uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask, uint64_t u0) {
+                                               uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask,
+                                               const uint32_t *lut, uint64_t u0) {
     using C = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
     const uint32_t lane = lane_id();
     uint64_t s[U];
@@ -279,16 +314,18 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
         s[u] = u0 + (uint64_t)(u * 64 + lane);
         cnt[u] = 0;
     }
-#pragma unroll 4
-    for (uint32_t i = 0; i < m; i++) {
-        const uint64_t k = kp[i] + sconst;
+    stream_keys(kp, 0, m, sconst, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t r = __umul24(fold_h16(k + s[u]), m) >> 16;
-            const uint32_t part = __umul24(r, magic) >> 20;
-            cnt[u] += (C)1 << __umul24(part, fb);
+            if (LUT) {
+                cnt[u] += lut[r];
+            } else {
+                const uint32_t part = __umul24(r, magic) >> 20;
+                cnt[u] += (C)1 << __umul24(part, fb);
+            }
         }
-    }
+    });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(((uint64_t)cnt[u] & cmask) == T);
@@ -300,6 +337,7 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 // Full leaf with rotation fitting (SIMDRecSplit.hpp:452-509): unit j is the base
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
+// The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
 template <int U>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
                                                    uint32_t nr, uint64_t u0) {
@@ -313,17 +351,14 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
         sb[u] = (u0 + (uint64_t)(u * 64 + lane)) * L;
         ml[u] = mr[u] = 0;
     }
-    for (uint32_t i = 0; i < L; i++) {
-        const uint64_t key = kp[i];
-        const uint64_t k = key + S;
-        if (key & 1) {
+    stream_keys(kp, 0, nl, S, [&](uint64_t k) {
 #pragma unroll
-            for (int u = 0; u < U; u++) mr[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
-        } else {
+        for (int u = 0; u < U; u++) ml[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+    });
+    stream_keys(kp, nl, L, S, [&](uint64_t k) {
 #pragma unroll
-            for (int u = 0; u < U; u++) ml[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
-        }
-    }
+        for (int u = 0; u < U; u++) mr[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+    });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
@@ -364,21 +399,19 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
         s[u] = u0 + (uint64_t)(u * 64 + lane);
         mk[u] = 0;
     }
-    for (uint32_t i = 0; i < mid; i++) {
-        const uint64_t k = kp[i] + S;
+    stream_keys(kp, 0, mid, S, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
-    }
+    });
     if (mid < m) {
         bool any = false;
 #pragma unroll
         for (int u = 0; u < U; u++) any |= (uint32_t)__popc(mk[u]) == mid;
         if (!__ballot(any)) return kNone;
-        for (uint32_t i = mid; i < m; i++) {
-            const uint64_t k = kp[i] + S;
+        stream_keys(kp, mid, m, S, [&](uint64_t k) {
 #pragma unroll
             for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
-        }
+        });
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -394,15 +427,18 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-template <int KIND, int U, bool WIDE>
+template <int KIND, int U, bool WIDE, bool LUT>
 __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
+    extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
+    uint32_t *lut = s_lut + (threadIdx.x >> 6) * a.lut_stride;
     int node = -1;
     bool helping = false;
     uint32_t scan = wave;
+    uint64_t work = 0;
     // cached per-node values (wave-uniform)
     uint64_t key_off = 0, sconst = 0, T = 0, cmask = 0;
     uint32_t m = 0, aux0 = 0, aux1 = 0;
@@ -451,6 +487,10 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 for (uint32_t p = 0; p + 1 < fan; p++) T |= (uint64_t)su << (p * a.field_bits);
                 const uint32_t w = (fan - 1) * a.field_bits;
                 cmask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+                if (LUT) {
+                    for (uint32_t r = lane; r < m; r += 64) lut[r] = 1u << (a.field_bits * (r / su));
+                    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                }
             } else if (KIND == kLeafRot) {
                 uint32_t odd = 0;
                 for (uint32_t i = 0; i < m; i++) odd += (uint32_t)(keys[key_off + i] & 1);
@@ -491,16 +531,20 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 r = uni64(r);
                 if (u0 * scale >= r) break;
             }
+            work += (uint64_t)U * 64 * m;
             if (KIND == kHigher) hit = group_higher<U>(kp, m, sconst, aux0, aux1, u0);
             else if (KIND == kUpper || KIND == kLower)
-                hit = group_aggr<U, WIDE>(kp, m, sconst, a.field_bits, a.magic, T, cmask, u0);
+                hit = group_aggr<U, WIDE, LUT>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut, u0);
             else if (KIND == kLeafRot) hit = group_leaf_rot<U>(kp, m, aux0, aux1, u0);
             else hit = group_leaf_brute<U>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
         if (hit != kNone && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)hit);
     }
-    if (lane == 0) __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) {
+        __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(a.work, (unsigned long long)work);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_state_init(NodeState *st, uint32_t n) {
@@ -521,8 +565,11 @@ __global__ __launch_bounds__(256) void k_finalize(const NodeDesc *__restrict__ n
     if (x >= kSeedLimit) atomicOr(err, 2u);
 }
 
-// Split nodes: record the seed and partition the node's keys in place by child
-// (stable), one wavefront per node, keys staged in LDS.
+// Split nodes: record the seed and partition the node's keys in place by child,
+// one wavefront per node, keys staged in LDS.  Inside every child the keys with
+// even `second` come first (stable within each parity): full leaves then split
+// into their left and right rotation sets by position.  Predicates are order-free,
+// so this layout does not change any seed.
 __global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
                                               const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
                                               uint32_t *__restrict__ err, SplitParams P, uint32_t kind) {
@@ -551,23 +598,47 @@ __global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const
         unit = kind == kUpper ? P.lower : P.leaf;
         fan = (m + unit - 1) / unit;
     }
-    uint32_t base[kMaxFanout];
+    auto child_of = [&](uint64_t key) -> uint32_t {
+        const uint32_t r = remap16(key + seed_abs, m);
+        return kind == kHigher ? (r >= unit ? 1u : 0u) : r / unit;
+    };
+    // pass 1: even keys per child
+    uint32_t evens[kMaxFanout];
 #pragma unroll
-    for (uint32_t c = 0; c < kMaxFanout; c++) base[c] = c * unit;
+    for (uint32_t c = 0; c < kMaxFanout; c++) evens[c] = 0;
+    for (uint32_t j0 = 0; j0 < m; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool valid = j < m;
+        const uint64_t key = valid ? s_keys[j] : 0;
+        const uint32_t cid = child_of(key);
+        const bool even = (key & 1) == 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kMaxFanout; c++)
+            if (c < fan) evens[c] += (uint32_t)__popcll(__ballot(valid && even && cid == c));
+    }
+    // pass 2: scatter
+    uint32_t base_e[kMaxFanout], base_o[kMaxFanout];
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxFanout; c++) {
+        base_e[c] = c * unit;
+        base_o[c] = c * unit + evens[c];
+    }
     const unsigned long long lt = (1ull << lane) - 1;
     for (uint32_t j0 = 0; j0 < m; j0 += 64) {
         const uint32_t j = j0 + lane;
         const bool valid = j < m;
         const uint64_t key = valid ? s_keys[j] : 0;
-        const uint32_t r = remap16(key + seed_abs, m);
-        const uint32_t cid = kind == kHigher ? (r >= unit ? 1u : 0u) : r / unit;
+        const uint32_t cid = child_of(key);
+        const bool even = (key & 1) == 0;
         uint32_t pos = 0;
 #pragma unroll
         for (uint32_t c = 0; c < kMaxFanout; c++) {
             if (c < fan) {
-                const unsigned long long msk = __ballot(valid && cid == c);
-                if (valid && cid == c) pos = base[c] + (uint32_t)__popcll(msk & lt);
-                base[c] += (uint32_t)__popcll(msk);
+                const unsigned long long me = __ballot(valid && even && cid == c);
+                const unsigned long long mo = __ballot(valid && !even && cid == c);
+                if (valid && cid == c) pos = even ? base_e[c] + (uint32_t)__popcll(me & lt) : base_o[c] + (uint32_t)__popcll(mo & lt);
+                base_e[c] += (uint32_t)__popcll(me);
+                base_o[c] += (uint32_t)__popcll(mo);
             }
         }
         if (valid) kp[pos] = key;

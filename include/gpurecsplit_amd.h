@@ -77,6 +77,7 @@ typedef struct rs_build_stats {
     double encode_ms;         /* Golomb-Rice + Elias-Fano */
     uint64_t evals[8];        /* algorithmic key-seed evaluations per kind (same index) */
     uint64_t nodes[8];        /* nodes per kind */
+    uint64_t executed_evals[8]; /* key-seed evaluations the kernels actually ran, per kind */
     uint64_t num_buckets;
     uint64_t num_nodes;
     uint64_t rice_bits;

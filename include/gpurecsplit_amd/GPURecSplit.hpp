@@ -1,0 +1,168 @@
+// GPURecSplit.hpp -- header-only C++ drop-in for the reference's GPURecSplit class,
+// backed by the gfx950 construction engine through the C ABI (gpurecsplit_amd.h).
+//
+// Reference surface (paths relative to the ByteHamster/GpuRecSplit root):
+//   template <size_t LEAF_SIZE, sux::util::AllocType AT, bool USE_BIJECTIONS_ROTATE>
+//   class GPURecSplit                         include/function/GPURecSplit.cuh:409-411
+//   GPURecSplit()                             :432
+//   GPURecSplit(vector<hash128_t>&, size_t bucket_size, int numThreads)   :484-489
+//   size_t operator()(const hash128_t&)       include/function/AbstractParallelRecSplit.hpp:409-466
+//   size_t getBits()                          AbstractParallelRecSplit.hpp:476-478
+//   operator<< / operator>>                   GPURecSplit.cuh:817-841
+//   hash128_t {first, second}                 include/util/Hash128.h:14-23
+//
+// Differences, all deliberate:
+//   - numThreads defaults to 1 (the two-argument form); construction runs on the GPU.
+//   - The caller's key vector is not reordered (the reference sorts it in place).
+//   - Failures throw gpurecsplit_amd::Error instead of exit()/abort() or looping
+//     forever on duplicate keys.
+//   - The string-key constructors (SpookyHash V2 front end) are not provided.
+//
+// Link with -lrecsplit_amd (gpurecsplit_amd/librecsplit_amd.so).  The GPU is the
+// one selected by the environment variable RS_DEVICE (default 0).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdlib>
+#include <istream>
+#include <iterator>
+#include <mutex>
+#include <ostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../gpurecsplit_amd.h"
+
+#if __has_include(<sux/util/Vector.hpp>)
+#include <sux/util/Vector.hpp>
+#else
+namespace sux::util {
+// Same enumerators as sux::util::AllocType; the engine keeps its own buffers, so
+// the value only selects the template instance.
+enum AllocType { MALLOC, SMALLPAGE, TRANSHUGEPAGE, FORCEHUGEPAGE };
+}  // namespace sux::util
+#endif
+
+namespace gpurecsplit_amd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+
+inline void check(int code, const char *where) {
+    if (code != RS_OK) throw Error(code, std::string(where) + ": " + rs_last_error());
+}
+
+// One build context per process and device, created on first use.
+inline rs_ctx *default_context() {
+    static std::once_flag once;
+    static rs_ctx *ctx = nullptr;
+    std::call_once(once, [] {
+        const char *d = std::getenv("RS_DEVICE");
+        check(rs_ctx_create(d ? std::atoi(d) : 0, &ctx), "rs_ctx_create");
+    });
+    return ctx;
+}
+
+}  // namespace gpurecsplit_amd
+
+namespace bez::function {
+
+#ifndef GPURECSPLIT_AMD_HASH128_DEFINED
+#define GPURECSPLIT_AMD_HASH128_DEFINED
+/** 128-bit key hash; `first` selects the bucket, `second` is hashed with the seeds. */
+typedef struct __hash128_t {
+    uint64_t first, second;
+
+    bool operator<(const __hash128_t &o) const { return first < o.first || second < o.second; }
+
+    __hash128_t() : first(0), second(0) {}
+    __hash128_t(const uint64_t first, const uint64_t second) {
+        this->first = first;
+        this->second = second;
+    }
+} hash128_t;
+static_assert(sizeof(hash128_t) == 16, "hash128_t must be two packed uint64");
+#endif
+
+template <size_t LEAF_SIZE, sux::util::AllocType AT = sux::util::AllocType::MALLOC, bool USE_BIJECTIONS_ROTATE = true>
+class GPURecSplit {
+    static_assert(LEAF_SIZE >= 2 && LEAF_SIZE <= 24, "leaf size must be in [2, 24]");
+    rs_mphf *h = nullptr;
+
+  public:
+    GPURecSplit() {}
+
+    /** Builds the MPHF of `keys` (distinct 128-bit hashes) with the given bucket size. */
+    GPURecSplit(std::vector<hash128_t> &keys, const size_t bucket_size, const int numThreads = 1) {
+        (void)numThreads;
+        gpurecsplit_amd::check(rs_build(gpurecsplit_amd::default_context(),
+                                        reinterpret_cast<const uint64_t *>(keys.data()), keys.size(), LEAF_SIZE,
+                                        bucket_size, USE_BIJECTIONS_ROTATE ? 1 : 0, &h),
+                               "GPURecSplit");
+    }
+
+    /** Builds from keys already resident in device memory of the context's GPU. */
+    static GPURecSplit fromDevice(const hash128_t *d_keys, size_t n, size_t bucket_size) {
+        GPURecSplit r;
+        gpurecsplit_amd::check(rs_build_device(gpurecsplit_amd::default_context(),
+                                               reinterpret_cast<const uint64_t *>(d_keys), n, LEAF_SIZE, bucket_size,
+                                               USE_BIJECTIONS_ROTATE ? 1 : 0, &r.h),
+                               "GPURecSplit::fromDevice");
+        return r;
+    }
+
+    GPURecSplit(const GPURecSplit &) = delete;
+    GPURecSplit &operator=(const GPURecSplit &) = delete;
+    GPURecSplit(GPURecSplit &&o) noexcept : h(o.h) { o.h = nullptr; }
+    GPURecSplit &operator=(GPURecSplit &&o) noexcept {
+        if (this != &o) {
+            rs_mphf_free(h);
+            h = o.h;
+            o.h = nullptr;
+        }
+        return *this;
+    }
+    ~GPURecSplit() { rs_mphf_free(h); }
+
+    /** Returns the value in [0, n) associated with the given 128-bit hash. */
+    size_t operator()(const hash128_t &hash) { return (size_t)rs_mphf_query(h, hash.first, hash.second); }
+
+    /** Space of the structure in bits (AbstractParallelRecSplit.hpp:476-478). */
+    size_t getBits() { return (size_t)rs_mphf_bits(h) + 8 * sizeof(*this); }
+
+    /** Number of keys of the function. */
+    size_t size() const { return h ? (size_t)rs_mphf_num_keys(h) : 0; }
+
+    /** Batched queries on the GPU: d_keys (n hashes) and d_out (n values) in device memory. */
+    void queryDevice(const hash128_t *d_keys, size_t n, uint64_t *d_out) const {
+        gpurecsplit_amd::check(rs_query_device(gpurecsplit_amd::default_context(), h,
+                                               reinterpret_cast<const uint64_t *>(d_keys), n, d_out),
+                               "GPURecSplit::queryDevice");
+    }
+
+    const rs_mphf *handle() const { return h; }
+
+    friend std::ostream &operator<<(std::ostream &os, const GPURecSplit &rs) {
+        std::vector<uint8_t> buf(rs_mphf_serialized_size(rs.h));
+        gpurecsplit_amd::check(rs_mphf_serialize(rs.h, buf.data(), buf.size()), "operator<<");
+        os.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)buf.size());
+        return os;
+    }
+
+    /** Reads one serialized structure (the rest of the stream is consumed). */
+    friend std::istream &operator>>(std::istream &is, GPURecSplit &rs) {
+        std::vector<uint8_t> buf((std::istreambuf_iterator<char>(is)), std::istreambuf_iterator<char>());
+        rs_mphf *nh = nullptr;
+        gpurecsplit_amd::check(rs_mphf_deserialize(buf.data(), buf.size(), LEAF_SIZE, USE_BIJECTIONS_ROTATE ? 1 : 0, &nh),
+                               "operator>>");
+        rs_mphf_free(rs.h);
+        rs.h = nh;
+        return is;
+    }
+};
+
+}  // namespace bez::function

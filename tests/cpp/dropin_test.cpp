@@ -1,0 +1,86 @@
+// Exercises the C++ drop-in header (include/gpurecsplit_amd/GPURecSplit.hpp) the
+// way the reference's benchmark and tests use GPURecSplit:
+//   dropin_test load  <mphf.bin> <keys.bin> <leaf>   operator>> + operator() bijection (no GPU)
+//   dropin_test build <keys.bin> <leaf> <bucket> <out.bin>
+//                                                   constructor + operator() + operator<< (GPU)
+// keys.bin holds n little-endian (first, second) uint64 pairs.
+#include <gpurecsplit_amd/GPURecSplit.hpp>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+using bez::function::GPURecSplit;
+using bez::function::hash128_t;
+
+static std::vector<hash128_t> read_keys(const char *path) {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<hash128_t> keys(raw.size() / 16);
+    std::memcpy(keys.data(), raw.data(), keys.size() * 16);
+    return keys;
+}
+
+template <class RS>
+static int check_bijection(RS &rs, const std::vector<hash128_t> &keys) {
+    // test/recsplitCorrectness.hpp:10-35, benchmark_construction.hpp:68-76
+    std::vector<bool> taken(keys.size(), false);
+    for (const auto &k : keys) {
+        const size_t v = rs(k);
+        if (v >= keys.size() || taken[v]) {
+            std::printf("collision or out of range for key (%llu, %llu): %zu\n", (unsigned long long)k.first,
+                        (unsigned long long)k.second, v);
+            return 1;
+        }
+        taken[v] = true;
+    }
+    return 0;
+}
+
+template <size_t L>
+static int run(int argc, char **argv) {
+    const std::string mode = argv[1];
+    if (mode == "load") {
+        GPURecSplit<L> rs;
+        std::ifstream in(argv[2], std::ios::binary);
+        in >> rs;
+        auto keys = read_keys(argv[3]);
+        if (check_bijection(rs, keys)) return 1;
+        std::printf("load ok: n=%zu bits/key=%.4f\n", keys.size(), (double)rs.getBits() / keys.size());
+        return 0;
+    }
+    auto keys = read_keys(argv[2]);
+    const auto before = keys;
+    GPURecSplit<L> rs(keys, std::stoul(argv[4]));
+    if (std::memcmp(before.data(), keys.data(), keys.size() * 16) != 0) {
+        std::printf("caller keys were modified\n");
+        return 1;
+    }
+    if (check_bijection(rs, keys)) return 1;
+    std::ofstream out(argv[5], std::ios::binary);
+    out << rs;
+    std::printf("build ok: n=%zu bits/key=%.4f\n", keys.size(), (double)rs.getBits() / keys.size());
+    (void)argc;
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: see source\n");
+        return 2;
+    }
+    const int leaf = std::stoi(std::string(argv[1]) == "load" ? argv[4] : argv[3]);
+    try {
+        switch (leaf) {
+        case 8: return run<8>(argc, argv);
+        case 12: return run<12>(argc, argv);
+        case 16: return run<16>(argc, argv);
+        default: std::fprintf(stderr, "leaf size %d not compiled in\n", leaf); return 2;
+        }
+    } catch (const gpurecsplit_amd::Error &e) {
+        std::printf("error %d: %s\n", e.code, e.what());
+        return 3;
+    }
+}

@@ -1,0 +1,44 @@
+"""The header-only C++ drop-in (include/gpurecsplit_amd/GPURecSplit.hpp), exercised
+by tests/cpp/dropin_test the way the reference's own tests use GPURecSplit:
+operator>> and operator() without a GPU, the constructor and operator<< with one."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleMPHF, splitmix_keys
+
+BIN = Path(__file__).resolve().parent / "cpp" / "dropin_test"
+
+
+def _run(*args):
+    return subprocess.run([str(BIN), *map(str, args)], capture_output=True, text=True, timeout=300)
+
+
+@pytest.mark.parametrize("leaf,bucket,n", [(8, 100, 20000), (16, 2000, 4000), (12, 100, 1)])
+def test_cpp_load_and_query(tmp_path, leaf, bucket, n):
+    keys = splitmix_keys(31 + leaf, n)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    (tmp_path / "mphf.bin").write_bytes(OracleMPHF.build(keys, leaf, bucket).serialize())
+    r = _run("load", tmp_path / "mphf.bin", tmp_path / "keys.bin", leaf)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "load ok" in r.stdout
+
+
+def test_cpp_load_rejects_leaf_mismatch(tmp_path):
+    keys = splitmix_keys(5, 1000)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    (tmp_path / "mphf.bin").write_bytes(OracleMPHF.build(keys, 8, 100).serialize())
+    r = _run("load", tmp_path / "mphf.bin", tmp_path / "keys.bin", 12)
+    assert r.returncode == 3 and "error 8" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("leaf,bucket,n", [(8, 100, 100000), (16, 2000, 20000)])
+def test_cpp_build_matches_oracle(tmp_path, leaf, bucket, n):
+    keys = splitmix_keys(77 + leaf, n)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    r = _run("build", tmp_path / "keys.bin", leaf, bucket, tmp_path / "out.bin")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "out.bin").read_bytes() == OracleMPHF.build(keys, leaf, bucket, threads=8).serialize()
