@@ -21,7 +21,7 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["GPURecSplit", "RecSplitError", "Context", "golomb_memo", "LIB_PATH", "STATUS"]
+__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "golomb_memo", "LIB_PATH", "STATUS"]
 
 LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
 
@@ -86,6 +86,8 @@ def _load():
         "rs_mphf_query": (C.c_uint64, [vp, C.c_uint64, C.c_uint64]),
         "rs_mphf_query_batch": (None, [vp, u64p, C.c_size_t, u64p]),
         "rs_query_device": (C.c_int, [vp, vp, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "rs_build_batch": (C.c_int, [vp, C.c_void_p, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
+                                     C.POINTER(vp), C.POINTER(BuildStats)]),
         "rs_mphf_bits": (C.c_uint64, [vp]),
         "rs_mphf_num_keys": (C.c_uint64, [vp]),
         "rs_mphf_stats": (C.c_int, [vp, C.POINTER(BuildStats)]),
@@ -244,3 +246,28 @@ class GPURecSplit:
         if getattr(self, "h", None):
             _free(self.h)
             self.h = None
+
+
+def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: bool = True,
+                ctx: Context | None = None):
+    """Builds one MPHF per key set in one batch on the GPU.
+
+    d_keys: contiguous (n, 2) 64-bit CUDA tensor holding all sets back to back;
+    set_offsets: num_sets + 1 increasing key offsets (set i = keys[off[i]:off[i+1]]).
+    Returns (list of GPURecSplit, batch stats dict).  Each result equals the
+    single-set build of its keys."""
+    ctx = ctx or Context.default()
+    if d_keys.dim() != 2 or d_keys.shape[1] != 2 or d_keys.element_size() != 8 or not d_keys.is_contiguous():
+        raise ValueError("d_keys must be a contiguous (n, 2) 64-bit CUDA tensor")
+    off = np.ascontiguousarray(set_offsets, dtype=np.uint64)
+    S = off.shape[0] - 1
+    if off[-1] > d_keys.shape[0]:
+        raise ValueError("set offsets exceed the key array")
+    _sync_torch(d_keys)
+    handles = (C.c_void_p * S)()
+    st = BuildStats()
+    _check(lib.rs_build_batch(ctx.h, C.c_void_p(d_keys.data_ptr()), off.ctypes.data_as(C.POINTER(C.c_uint64)), S,
+                              leaf_size, bucket_size, int(rotate), handles, C.byref(st)))
+    out = [GPURecSplit(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=C.c_void_p(h))
+           for h in handles]
+    return out, st.as_dict()

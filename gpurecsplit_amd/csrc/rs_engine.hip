@@ -226,7 +226,7 @@ struct rs_ctx {
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
         list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
-        ef_up, ef_jump;
+        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off;
     uint32_t *memo[kMaxLeaf + 1] = {};
     uint32_t *h_pinned = nullptr;  // small pinned staging area
     hipEvent_t ev[128];
@@ -344,39 +344,69 @@ static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, const 
     HIP_CHECK(hipGetLastError());
 }
 
-static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t leaf, uint64_t bucket_size,
-                      bool rotate, const std::chrono::steady_clock::time_point t_start) {
+__global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *__restrict__ idx, uint64_t n,
+                             uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+constexpr uint64_t kTileKeys = 1 << 14;  // keys per histogram / scatter workgroup
+
+// Builds one MPHF per key set; set s owns keys [set_off[s], set_off[s+1]) of d_keys.
+// All sets share one leaf size and bucket size; their buckets, tree nodes and
+// seed searches run together.  With device_copy, the (single) result also keeps
+// a device-resident copy for batched GPU queries.
+static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, const std::vector<uint64_t> &set_off,
+                                         uint32_t leaf, uint64_t bucket_size, bool rotate, bool device_copy,
+                                         const std::chrono::steady_clock::time_point t_start) {
     hipStream_t s = c->stream;
     c->n_ev = 0;
+    const size_t S = set_off.size() - 1;
     const SplitParams P = split_params(leaf);
     const uint32_t *memo = ctx_memo(c, leaf);
     const Memo &hmemo = golomb_memo(leaf);
-    const uint64_t nb = std::max<uint64_t>(1, (n + bucket_size - 1) / bucket_size);
     rs_build_stats st{};
-    st.num_buckets = nb;
+
+    // ---- sets, buckets and key tiles
+    std::vector<uint64_t> nbs(S), bbase(S + 1);
+    std::vector<KeyTile> tiles;
+    uint64_t NB = 0, max_nb = 0;
+    for (size_t i = 0; i < S; i++) {
+        const uint64_t n = set_off[i + 1] - set_off[i];
+        nbs[i] = std::max<uint64_t>(1, (n + bucket_size - 1) / bucket_size);
+        bbase[i] = NB;
+        NB += nbs[i];
+        max_nb = std::max(max_nb, nbs[i]);
+        for (uint64_t k = set_off[i]; k < set_off[i + 1]; k += kTileKeys)
+            tiles.push_back(KeyTile{k, std::min(k + kTileKeys, set_off[i + 1]), nbs[i], bbase[i]});
+    }
+    bbase[S] = NB;
+    const uint64_t n_total = set_off[S] - set_off[0];
+    st.num_buckets = NB;
 
     // ---- 1. bucket histogram and per-bucket metadata
     const int e_part0 = ev_record(c);
-    uint32_t *counts = c->counts.get<uint32_t>(nb);
-    uint64_t *counts64 = c->counts64.get<uint64_t>(nb);
-    uint64_t *acc = c->acc.get<uint64_t>(nb + 1);
-    uint64_t *node_cnt = c->node_cnt.get<uint64_t>(nb);
-    uint64_t *node_base = c->node_base.get<uint64_t>(nb + 1);
+    KeyTile *d_tiles = c->tiles.get<KeyTile>(tiles.size());
+    HIP_CHECK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(KeyTile), hipMemcpyHostToDevice, s));
+    uint32_t *counts = c->counts.get<uint32_t>(NB);
+    uint64_t *counts64 = c->counts64.get<uint64_t>(NB);
+    uint64_t *acc = c->acc.get<uint64_t>(NB + 1);
+    uint64_t *node_cnt = c->node_cnt.get<uint64_t>(NB);
+    uint64_t *node_base = c->node_base.get<uint64_t>(NB + 1);
     uint32_t *size_hist = c->size_hist.get<uint32_t>(kMaxBucket + 1);
     uint32_t *err = c->err.get<uint32_t>(4);
-    HIP_CHECK(hipMemsetAsync(counts, 0, nb * 4, s));
+    HIP_CHECK(hipMemsetAsync(counts, 0, NB * 4, s));
     HIP_CHECK(hipMemsetAsync(size_hist, 0, (kMaxBucket + 1) * 4, s));
     HIP_CHECK(hipMemsetAsync(err, 0, 16, s));
-    const uint32_t hist_blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c->num_cus * 8);
-    if (nb <= 8192)
-        k_bucket_hist<true><<<hist_blocks, 256, nb * 4, s>>>(d_keys, n, nb, counts);
+    if (max_nb <= 8192)
+        k_bucket_hist<true><<<tiles.size(), 256, max_nb * 4, s>>>(d_keys, d_tiles, counts);
     else
-        k_bucket_hist<false><<<hist_blocks, 256, 0, s>>>(d_keys, n, nb, counts);
+        k_bucket_hist<false><<<tiles.size(), 256, 0, s>>>(d_keys, d_tiles, counts);
     HIP_CHECK(hipGetLastError());
-    k_bucket_meta<<<(nb + 255) / 256, 256, 0, s>>>(counts, nb, memo, counts64, node_cnt, size_hist);
+    k_bucket_meta<<<(NB + 255) / 256, 256, 0, s>>>(counts, NB, memo, counts64, node_cnt, size_hist);
     HIP_CHECK(hipGetLastError());
-    scan_u64(c, counts, true, nb, acc);
-    scan_u64(c, node_cnt, false, nb, node_base);
+    scan_u64(c, counts, true, NB, acc);
+    scan_u64(c, node_cnt, false, NB, node_base);
     uint32_t *h = c->h_pinned;
     HIP_CHECK(hipMemcpyAsync(h, size_hist, (kMaxBucket + 1) * 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -399,18 +429,15 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
         total_nodes += cen.count[l];
     }
     st.num_nodes = total_nodes;
-    for (int l = 0; l < kNumLists; l++) {
-        const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
-        st.nodes[kind] += cen.count[l];
-    }
+    for (int l = 0; l < kNumLists; l++) st.nodes[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += cen.count[l];
 
     // ---- 2. scatter and per-bucket sort
-    uint64_t *sorted = c->sorted.get<uint64_t>(n + 8);  // +8: look-ahead loads of the search
-    uint32_t *cursor = c->cursor.get<uint32_t>(nb);
-    HIP_CHECK(hipMemsetAsync(cursor, 0, nb * 4, s));
-    k_scatter<<<hist_blocks, 256, 0, s>>>(d_keys, n, nb, acc, cursor, sorted);
+    uint64_t *sorted = c->sorted.get<uint64_t>(n_total + 8);  // +8: look-ahead loads of the search
+    uint32_t *cursor = c->cursor.get<uint32_t>(NB);
+    HIP_CHECK(hipMemsetAsync(cursor, 0, NB * 4, s));
+    k_scatter<<<tiles.size(), 256, 0, s>>>(d_keys, d_tiles, acc, cursor, sorted);
     HIP_CHECK(hipGetLastError());
-    k_sort_buckets<<<nb, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err, rotate ? leaf : 0);
+    k_sort_buckets<<<NB, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err, rotate ? leaf : 0);
     HIP_CHECK(hipGetLastError());
 
     // ---- 3. node lists
@@ -423,7 +450,7 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     HIP_CHECK(hipMemcpyAsync(d_lbase, lbase.data(), kNumLists * 8, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemsetAsync(d_lfill, 0, kNumLists * 4, s));
     ExpandArgs ea{P, rotate ? 1 : 0, pool, d_lbase, d_lfill, node_m};
-    k_expand<<<(nb + 63) / 64, 64, 0, s>>>(acc, node_base, nb, ea);
+    k_expand<<<(NB + 63) / 64, 64, 0, s>>>(acc, node_base, NB, ea);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -494,112 +521,133 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     }
     const int e_search_end = ev_record(c);
 
-    // ---- 5. descriptor lengths
-    uint64_t *bits = c->bits.get<uint64_t>(nb);
-    uint64_t *pos = c->pos.get<uint64_t>(nb + 1);
+    // ---- 5. descriptor lengths (global scan; each set reads its segment)
+    uint64_t *bits = c->bits.get<uint64_t>(NB);
+    uint64_t *pos = c->pos.get<uint64_t>(NB + 1);
     unsigned long long *evals = c->evals.get<unsigned long long>(kNumKinds);
     HIP_CHECK(hipMemsetAsync(evals, 0, kNumKinds * 8, s));
-    EncodeArgs ca{acc, node_base, seeds, node_m, memo, nb, P, rotate ? 1 : 0};
-    k_bucket_bits<<<(nb + 3) / 4, 256, 0, s>>>(ca, bits, evals);
+    EncodeArgs ca{acc, node_base, seeds, node_m, memo, NB, P, rotate ? 1 : 0};
+    k_bucket_bits<<<(NB + 3) / 4, 256, 0, s>>>(ca, bits, evals);
     HIP_CHECK(hipGetLastError());
-    scan_u64(c, bits, false, nb, pos);
+    scan_u64(c, bits, false, NB, pos);
+    uint64_t *d_bidx = c->set_idx.get<uint64_t>(S + 1);
+    uint64_t *d_gath = c->set_gather.get<uint64_t>(S + 1);
+    HIP_CHECK(hipMemcpyAsync(d_bidx, bbase.data(), (S + 1) * 8, hipMemcpyHostToDevice, s));
+    k_gather_u64<<<(S + 1 + 255) / 256, 256, 0, s>>>(pos, d_bidx, S + 1, d_gath);
+    HIP_CHECK(hipGetLastError());
+    std::vector<uint64_t> pos_at(S + 1);
     uint64_t *h64 = (uint64_t *)c->h_pinned;
-    HIP_CHECK(hipMemcpyAsync(h64, pos + nb, 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pos_at.data(), d_gath, (S + 1) * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h64 + 1, evals, kNumKinds * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h64 + 16, work, kNumLists * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    const uint64_t total_bits = h64[0];
     for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
     for (int l = 0; l < kNumLists; l++)
         st.executed_evals[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += h64[16 + l];
     if (h[16] & 2) throw BuildError(RS_ERR_SEED_OVERFLOW, "a node search passed the 2^44 seed limit");
-    const uint64_t B = total_bits + 1;  // with the sentinel
-    const uint64_t rice_words = ((B + 7) / 8 + 14) / 8;
-    st.rice_bits = B;
 
-    // ---- 6. Golomb-Rice and Elias-Fano
-    unsigned long long *rice = c->rice.get<unsigned long long>(rice_words + 2);
-    HIP_CHECK(hipMemsetAsync(rice, 0, (rice_words + 2) * 8, s));
-    k_rice_write<<<(nb + 3) / 4, 256, 0, s>>>(ca, pos, rice);
-    k_rice_sentinel<<<1, 1, 0, s>>>(rice, pos, nb);
-    EfScalars *efs = c->ef_scalars.get<EfScalars>(1);
-    k_ef_scalars<<<1, 1024, 0, s>>>(acc, pos, nb, efs);
+    // ---- 6. Golomb-Rice: per-set vectors packed one after another
+    std::vector<SetEnc> enc(S);
+    std::vector<uint64_t> rice_words(S);
+    uint64_t rice_total = 0;
+    for (size_t i = 0; i < S; i++) {
+        const uint64_t B = pos_at[i + 1] - pos_at[i] + 1;  // with the sentinel
+        rice_words[i] = ((B + 7) / 8 + 14) / 8;
+        enc[i].bbase = bbase[i];
+        enc[i].nb = nbs[i];
+        enc[i].rice_word = rice_total;
+        rice_total += rice_words[i];
+        st.rice_bits += B;
+    }
+    SetEnc *d_enc = c->set_enc.get<SetEnc>(S);
+    HIP_CHECK(hipMemcpyAsync(d_enc, enc.data(), S * sizeof(SetEnc), hipMemcpyHostToDevice, s));
+    unsigned long long *rice = c->rice.get<unsigned long long>(rice_total + 2);
+    long long *bit_off = c->bit_off.get<long long>(NB);
+    HIP_CHECK(hipMemsetAsync(rice, 0, (rice_total + 2) * 8, s));
+    k_set_offsets<<<S, 256, 0, s>>>(d_enc, pos, bit_off, rice);
+    k_rice_write<<<(NB + 3) / 4, 256, 0, s>>>(ca, pos, bit_off, rice);
+    EfScalars *d_efs = c->ef_scalars.get<EfScalars>(S);
+    k_ef_scalars<<<S, 256, 0, s>>>(d_enc, acc, pos, d_efs);
     HIP_CHECK(hipGetLastError());
-    EfScalars he;
-    HIP_CHECK(hipMemcpyAsync(h64, efs, sizeof(EfScalars), hipMemcpyDeviceToHost, s));
+    std::vector<EfScalars> he(S);
+    HIP_CHECK(hipMemcpyAsync(he.data(), d_efs, S * sizeof(EfScalars), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    memcpy(&he, h64, sizeof(he));
-    EliasFano ef;
-    ef.num_buckets = nb;
-    ef.bits_per_key_fixed_point = he.bpk;
-    ef.cum_keys_min_delta = he.min_delta;
-    ef.min_diff = he.min_diff;
-    // DoubleEF.hpp:181-184
-    ef.u_position = (uint64_t)((int64_t)total_bits - (int64_t)(he.bpk * n >> 20) - (int64_t)(nb * (uint64_t)he.min_diff) + 1);
-    ef.u_cum_keys = n - nb * (uint64_t)he.min_delta + 1;
-    ef.derive_widths();
-    const uint64_t n_lower = ef_lower_words(nb, ef.l_cum_keys, ef.l_position);
-    const uint64_t n_uc = ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys);
-    const uint64_t n_up = ef_upper_words(nb, ef.u_position, ef.l_position);
-    const uint64_t n_jump = ef_jump_words(nb);
-    unsigned long long *d_lower = c->ef_lower.get<unsigned long long>(n_lower + 1);
-    unsigned long long *d_uc = c->ef_uc.get<unsigned long long>(n_uc + 1);
-    unsigned long long *d_up = c->ef_up.get<unsigned long long>(n_up + 1);
-    uint8_t *d_jump = c->ef_jump.get<uint8_t>(n_jump * 8 + 8);
-    HIP_CHECK(hipMemsetAsync(d_lower, 0, (n_lower + 1) * 8, s));
-    HIP_CHECK(hipMemsetAsync(d_uc, 0, (n_uc + 1) * 8, s));
-    HIP_CHECK(hipMemsetAsync(d_up, 0, (n_up + 1) * 8, s));
-    HIP_CHECK(hipMemsetAsync(d_jump, 0, n_jump * 8 + 8, s));
-    EfFill ff{acc, pos, nb, he.bpk, ef.l_cum_keys, ef.l_position, he.min_delta, he.min_diff, d_lower, d_uc, d_up,
-              d_jump, n_jump * 8, err};
-    k_ef_fill<<<(nb + 1 + 255) / 256, 256, 0, s>>>(ff);
+
+    // ---- 7. Elias-Fano: per-set widths and sizes (DoubleEF.hpp:181-202)
+    std::vector<EliasFano> efs(S);
+    uint64_t tot_lower = 0, tot_uc = 0, tot_up = 0, tot_jump = 0;
+    for (size_t i = 0; i < S; i++) {
+        EliasFano &ef = efs[i];
+        const uint64_t nb = nbs[i], n = set_off[i + 1] - set_off[i];
+        ef.num_buckets = nb;
+        ef.bits_per_key_fixed_point = he[i].bpk;
+        ef.cum_keys_min_delta = he[i].min_delta;
+        ef.min_diff = he[i].min_diff;
+        ef.u_position = (uint64_t)((int64_t)he[i].total_bits - (int64_t)(he[i].bpk * n >> 20) -
+                                   (int64_t)(nb * (uint64_t)he[i].min_diff) + 1);
+        ef.u_cum_keys = n - nb * (uint64_t)he[i].min_delta + 1;
+        ef.derive_widths();
+        enc[i].lc = ef.l_cum_keys;
+        enc[i].lp = ef.l_position;
+        enc[i].lower_word = tot_lower;
+        enc[i].uc_word = tot_uc;
+        enc[i].up_word = tot_up;
+        enc[i].jump_word = tot_jump;
+        enc[i].jump_words = ef_jump_words(nb);
+        ef.lower_bits.assign(ef_lower_words(nb, ef.l_cum_keys, ef.l_position), 0);
+        ef.upper_cum.assign(ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys), 0);
+        ef.upper_pos.assign(ef_upper_words(nb, ef.u_position, ef.l_position), 0);
+        ef.jump.assign(enc[i].jump_words, 0);
+        tot_lower += ef.lower_bits.size();
+        tot_uc += ef.upper_cum.size();
+        tot_up += ef.upper_pos.size();
+        tot_jump += ef.jump.size();
+    }
+    HIP_CHECK(hipMemcpyAsync(d_enc, enc.data(), S * sizeof(SetEnc), hipMemcpyHostToDevice, s));
+    unsigned long long *d_lower = c->ef_lower.get<unsigned long long>(tot_lower + 1);
+    unsigned long long *d_uc = c->ef_uc.get<unsigned long long>(tot_uc + 1);
+    unsigned long long *d_up = c->ef_up.get<unsigned long long>(tot_up + 1);
+    uint8_t *d_jump = c->ef_jump.get<uint8_t>(tot_jump * 8 + 8);
+    HIP_CHECK(hipMemsetAsync(d_lower, 0, (tot_lower + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_uc, 0, (tot_uc + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_up, 0, (tot_up + 1) * 8, s));
+    HIP_CHECK(hipMemsetAsync(d_jump, 0, tot_jump * 8 + 8, s));
+    k_ef_fill<<<S, 256, 0, s>>>(d_enc, d_efs, acc, pos, d_lower, d_uc, d_up, d_jump, err);
     HIP_CHECK(hipGetLastError());
     const int e_enc_end = ev_record(c);
 
-    // ---- 7. results to the host
-    rs_mphf *out = new rs_mphf();
-    out->leaf = leaf;
-    out->rotate = rotate ? 1 : 0;
-    out->bucket_size = bucket_size;
-    out->keys_count = n;
-    out->nbuckets = nb;
-    out->rice.assign(rice_words + 2, 0);
-    out->rice_words = rice_words;
-    ef.lower_bits.assign(n_lower, 0);
-    ef.upper_cum.assign(n_uc, 0);
-    ef.upper_pos.assign(n_up, 0);
-    ef.jump.assign(n_jump, 0);
-    HIP_CHECK(hipMemcpyAsync(out->rice.data(), rice, rice_words * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(ef.lower_bits.data(), d_lower, n_lower * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(ef.upper_cum.data(), d_uc, n_uc * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(ef.upper_pos.data(), d_up, n_up * 8, hipMemcpyDeviceToHost, s));
-    if (n_jump) HIP_CHECK(hipMemcpyAsync(ef.jump.data(), d_jump, n_jump * 8, hipMemcpyDeviceToHost, s));
+    // ---- 8. results to the host
+    std::vector<uint64_t> h_rice(rice_total), h_lower(tot_lower), h_uc(tot_uc), h_up(tot_up), h_jump(tot_jump);
+    HIP_CHECK(hipMemcpyAsync(h_rice.data(), rice, rice_total * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_lower.data(), d_lower, tot_lower * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_uc.data(), d_uc, tot_uc * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_up.data(), d_up, tot_up * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_jump.data(), d_jump, tot_jump * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
-    // device copy for batched queries
-    DeviceCopy *dc = new DeviceCopy();
-    dc->device = c->device;
-    out->dev = dc;
-    HIP_CHECK(hipMalloc(&dc->rice, (rice_words + 2) * 8));
-    HIP_CHECK(hipMalloc(&dc->lower, (n_lower + 1) * 8));
-    HIP_CHECK(hipMalloc(&dc->upper_c, (n_uc + 1) * 8));
-    HIP_CHECK(hipMalloc(&dc->upper_p, (n_up + 1) * 8));
-    HIP_CHECK(hipMalloc(&dc->jump, n_jump * 8 + 8));
-    HIP_CHECK(hipMalloc(&dc->memo, kMaxBucket * 4));
-    HIP_CHECK(hipMemcpyAsync(dc->rice, rice, (rice_words + 2) * 8, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(dc->lower, d_lower, (n_lower + 1) * 8, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(dc->upper_c, d_uc, (n_uc + 1) * 8, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (n_up + 1) * 8, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, n_jump * 8 + 8, hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(dc->memo, memo, kMaxBucket * 4, hipMemcpyDeviceToDevice, s));
+    DeviceCopy *dc = nullptr;
+    if (device_copy && S == 1) {
+        dc = new DeviceCopy();
+        dc->device = c->device;
+        const EliasFano &ef = efs[0];
+        HIP_CHECK(hipMalloc(&dc->rice, (rice_total + 2) * 8));
+        HIP_CHECK(hipMalloc(&dc->lower, (ef.lower_bits.size() + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->upper_c, (ef.upper_cum.size() + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->upper_p, (ef.upper_pos.size() + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->jump, ef.jump.size() * 8 + 8));
+        HIP_CHECK(hipMalloc(&dc->memo, kMaxBucket * 4));
+        HIP_CHECK(hipMemcpyAsync(dc->rice, rice, (rice_total + 2) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->lower, d_lower, (ef.lower_bits.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->upper_c, d_uc, (ef.upper_cum.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (ef.upper_pos.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, ef.jump.size() * 8 + 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->memo, memo, kMaxBucket * 4, hipMemcpyDeviceToDevice, s));
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[16] & 4) {
-        rs_mphf_free(out);
+        if (dc) device_copy_free(dc);
         throw BuildError(RS_ERR_EF_OVERFLOW, "Elias-Fano jump offset does not fit 16 bits");
     }
-    out->ef = std::move(ef);
-    out->memo = &hmemo;
-    out->P = P;
 
     // ---- timings
     st.partition_ms = ev_ms(c, e_part0, e_part1);
@@ -609,8 +657,37 @@ static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t le
     }
     st.encode_ms = ev_ms(c, e_search_end, e_enc_end);
     st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    out->stats = st;
+
+    std::vector<rs_mphf *> out(S);
+    for (size_t i = 0; i < S; i++) {
+        rs_mphf *r = new rs_mphf();
+        r->leaf = leaf;
+        r->rotate = rotate ? 1 : 0;
+        r->bucket_size = bucket_size;
+        r->keys_count = set_off[i + 1] - set_off[i];
+        r->nbuckets = nbs[i];
+        r->rice_words = rice_words[i];
+        r->rice.assign(h_rice.begin() + enc[i].rice_word, h_rice.begin() + enc[i].rice_word + rice_words[i]);
+        r->rice.resize(rice_words[i] + 2, 0);
+        EliasFano &ef = efs[i];
+        std::copy(h_lower.begin() + enc[i].lower_word, h_lower.begin() + enc[i].lower_word + ef.lower_bits.size(),
+                  ef.lower_bits.begin());
+        std::copy(h_uc.begin() + enc[i].uc_word, h_uc.begin() + enc[i].uc_word + ef.upper_cum.size(), ef.upper_cum.begin());
+        std::copy(h_up.begin() + enc[i].up_word, h_up.begin() + enc[i].up_word + ef.upper_pos.size(), ef.upper_pos.begin());
+        std::copy(h_jump.begin() + enc[i].jump_word, h_jump.begin() + enc[i].jump_word + ef.jump.size(), ef.jump.begin());
+        r->ef = std::move(ef);
+        r->memo = &hmemo;
+        r->P = P;
+        r->stats = st;
+        if (i == 0) r->dev = dc;
+        out[i] = r;
+    }
     return out;
+}
+
+static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t leaf, uint64_t bucket_size,
+                      bool rotate, const std::chrono::steady_clock::time_point t_start) {
+    return build_sets(c, d_keys, {0, n}, leaf, bucket_size, rotate, true, t_start)[0];
 }
 
 static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn) {
@@ -678,7 +755,8 @@ void rs_ctx_destroy(rs_ctx *c) {
     for (DevBuf *b : {&c->keys_in, &c->counts, &c->counts64, &c->cursor, &c->acc, &c->node_cnt, &c->node_base,
                       &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
                       &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
-                      &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump})
+                      &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
+                      &c->set_gather, &c->set_enc, &c->bit_off})
         b->release();
     for (auto &m : c->memo)
         if (m) (void)hipFree(m);
@@ -702,6 +780,21 @@ int rs_build(rs_ctx *c, const uint64_t *keys, size_t n, uint32_t leaf, size_t bu
         uint64_t *d = c->keys_in.get<uint64_t>(2 * n);
         HIP_CHECK(hipMemcpyAsync(d, keys, n * 16, hipMemcpyHostToDevice, c->stream));
         *out = build(c, d, n, leaf, bucket, rotate != 0, t0);
+    });
+}
+
+int rs_build_batch(rs_ctx *c, const uint64_t *d_keys, const uint64_t *set_offsets, size_t num_sets, uint32_t leaf,
+                   size_t bucket, int rotate, rs_mphf **out, rs_build_stats *stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!set_offsets || num_sets == 0) return set_error("rs_build_batch: no key sets"), RS_ERR_ARG;
+    for (size_t i = 0; i < num_sets; i++)
+        if (set_offsets[i + 1] <= set_offsets[i]) return set_error("rs_build_batch: every key set needs a key"), RS_ERR_ARG;
+    if (int e = check_args(c, d_keys, set_offsets[num_sets] - set_offsets[0], leaf, bucket, out)) return e;
+    return guarded("rs_build_batch", c, [&] {
+        std::vector<uint64_t> off(set_offsets, set_offsets + num_sets + 1);
+        std::vector<rs_mphf *> r = build_sets(c, d_keys, off, leaf, bucket, rotate != 0, false, t0);
+        for (size_t i = 0; i < num_sets; i++) out[i] = r[i];
+        if (stats) *stats = r[0]->stats;
     });
 }
 

@@ -65,25 +65,34 @@ __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long lon
 
 // ------------------------------------------------------------------ partition
 
-// Bucket histogram; bucket = floor(first * nb / 2^64) (common.hpp:273-280).
+// A contiguous range of keys of one key set: its keys map to buckets
+// bbase + floor(first * nb / 2^64) (common.hpp:273-280, with the set's own nb).
+struct KeyTile {
+    uint64_t k0, k1;  // key range [k0, k1)
+    uint64_t nb;      // buckets of the set
+    uint64_t bbase;   // first global bucket of the set
+};
+
+// Bucket histogram, one workgroup per key tile.  Sets with few buckets count in
+// LDS first (PRIV: every tile's set has at most 8192 buckets).
 template <bool PRIV>
-__global__ __launch_bounds__(256) void k_bucket_hist(const uint64_t *__restrict__ keys, uint64_t n, uint64_t nb,
+__global__ __launch_bounds__(256) void k_bucket_hist(const uint64_t *__restrict__ keys, const KeyTile *__restrict__ tiles,
                                                      uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t s_cnt[];
+    const KeyTile t = tiles[blockIdx.x];
     if (PRIV) {
-        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_cnt[i] = 0;
+        for (uint32_t i = threadIdx.x; i < t.nb; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
     }
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = __umul64hi(keys[2 * i], nb);
+    for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) {
+        const uint64_t b = __umul64hi(keys[2 * i], t.nb);
         if (PRIV) atomicAdd(&s_cnt[b], 1u);
-        else atomicAdd(&counts[b], 1u);
+        else atomicAdd(&counts[t.bbase + b], 1u);
     }
     if (PRIV) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
-            if (s_cnt[i]) atomicAdd(&counts[i], s_cnt[i]);
+        for (uint32_t i = threadIdx.x; i < t.nb; i += blockDim.x)
+            if (s_cnt[i]) atomicAdd(&counts[t.bbase + i], s_cnt[i]);
     }
 }
 
@@ -104,12 +113,12 @@ __global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict_
     nodes[b] = (memo[m] >> 16) & 0x7FF;
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const uint64_t *__restrict__ keys, uint64_t n, uint64_t nb,
+__global__ __launch_bounds__(256) void k_scatter(const uint64_t *__restrict__ keys, const KeyTile *__restrict__ tiles,
                                                  const uint64_t *__restrict__ bucket_acc, uint32_t *__restrict__ cursor,
                                                  uint64_t *__restrict__ sorted) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = __umul64hi(keys[2 * i], nb);
+    const KeyTile t = tiles[blockIdx.x];
+    for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) {
+        const uint64_t b = t.bbase + __umul64hi(keys[2 * i], t.nb);
         const uint32_t p = atomicAdd(&cursor[b], 1u);
         sorted[bucket_acc[b] + p] = keys[2 * i + 1];
     }
@@ -723,6 +732,7 @@ __device__ __forceinline__ T wave_excl_scan(T v, T &total) {
 // Per bucket (one wavefront): write fixed parts in DFS order, then the unary
 // parts (RiceBitVector.hpp:73-106, SIMDRecSplit.hpp:707-714).
 __global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t *__restrict__ pos,
+                                                    const long long *__restrict__ bit_off,
                                                     unsigned long long *__restrict__ words) {
     const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = lane_id();
@@ -731,7 +741,7 @@ __global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t
     if (m < 2) return;
     const uint64_t base = a.node_base[b];
     const uint32_t nn = (a.memo[m] >> 16) & 0x7FF;
-    const uint64_t p0 = pos[b];
+    const uint64_t p0 = pos[b] + (uint64_t)bit_off[b];
     const uint64_t fixed_total = a.memo[m] & 0xFFFF;
     uint64_t fpre = 0, upre = 0;
     for (uint32_t j0 = 0; j0 < nn; j0 += 64) {
@@ -757,12 +767,29 @@ __global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t
     }
 }
 
-__global__ void k_rice_sentinel(unsigned long long *words, const uint64_t *pos, uint64_t nb) {
-    const uint64_t p = pos[nb];
-    atomicOr(&words[p >> 6], 1ull << (p & 63));
-}
+// Per-set layout of the encoded structures (filled by the host between phases).
+struct SetEnc {
+    uint64_t bbase, nb;          // global bucket range [bbase, bbase + nb)
+    uint64_t rice_word;          // first word of the set's Golomb-Rice vector
+    uint64_t lower_word, uc_word, up_word, jump_word;  // EF arrays
+    uint64_t jump_words;         // words of the set's jump table
+    uint64_t lc, lp;             // EF lower-bit widths
+};
 
-// ------------------------------------------------------------------ Elias-Fano
+// One workgroup per set: bit offset of every bucket of the set inside the global
+// Golomb-Rice buffer, and the sentinel bit after the set's last bucket
+// (appendFixed(1, 1), SIMDRecSplit.hpp:810).
+__global__ __launch_bounds__(256) void k_set_offsets(const SetEnc *__restrict__ sets, const uint64_t *__restrict__ pos,
+                                                     long long *__restrict__ bit_off,
+                                                     unsigned long long *__restrict__ words) {
+    const SetEnc e = sets[blockIdx.x];
+    const long long off = (long long)(e.rice_word * 64) - (long long)pos[e.bbase];
+    for (uint64_t i = threadIdx.x; i < e.nb; i += blockDim.x) bit_off[e.bbase + i] = off;
+    if (threadIdx.x == 0) {
+        const uint64_t p = pos[e.bbase + e.nb] - pos[e.bbase] + e.rice_word * 64;
+        atomicOr(&words[p >> 6], 1ull << (p & 63));
+    }
+}
 
 struct EfScalars {
     uint64_t bpk;
@@ -771,21 +798,26 @@ struct EfScalars {
     uint64_t total_bits;
 };
 
-// One workgroup: bits_per_key_fixed_point (IEEE double, DoubleEF.hpp:144) and the
-// two minima (DoubleEF.hpp:146-179).
-__global__ __launch_bounds__(1024) void k_ef_scalars(const uint64_t *__restrict__ cum, const uint64_t *__restrict__ pos,
-                                                     uint64_t nb, EfScalars *out) {
+// One workgroup per set: bits_per_key_fixed_point (IEEE double, DoubleEF.hpp:144)
+// and the two minima (DoubleEF.hpp:146-179) over the set's buckets, with key
+// counts and bit positions relative to the set.
+__global__ __launch_bounds__(256) void k_ef_scalars(const SetEnc *__restrict__ sets, const uint64_t *__restrict__ acc,
+                                                    const uint64_t *__restrict__ pos, EfScalars *__restrict__ out) {
     __shared__ uint64_t s_bpk;
-    __shared__ long long s_a[1024], s_b[1024];
-    if (threadIdx.x == 0) s_bpk = (uint64_t)((double)(1ull << 20) * ((double)pos[nb] / (double)cum[nb]));
+    __shared__ long long s_a[256], s_b[256];
+    const SetEnc e = sets[blockIdx.x];
+    const uint64_t *cum = acc + e.bbase, *ps = pos + e.bbase;
+    const uint64_t c0 = cum[0], p0 = ps[0], nb = e.nb;
+    if (threadIdx.x == 0) s_bpk = (uint64_t)((double)(1ull << 20) * ((double)(ps[nb] - p0) / (double)(cum[nb] - c0)));
     __syncthreads();
     const uint64_t bpk = s_bpk;
     long long md = 0x3FFFFFFFFFFFFFFFll, mf = 0x3FFFFFFFFFFFFFFFll;
     for (uint64_t i = 1 + threadIdx.x; i <= nb; i += blockDim.x) {
-        const long long d = (long long)(cum[i] - cum[i - 1]);
+        const uint64_t ci = cum[i] - c0, cp = cum[i - 1] - c0;
+        const long long d = (long long)(ci - cp);
         md = d < md ? d : md;
-        const long long bb = (long long)pos[i] - (long long)(bpk * cum[i] >> 20);
-        const long long bp = (long long)pos[i - 1] - (long long)(bpk * cum[i - 1] >> 20);
+        const long long bb = (long long)(ps[i] - p0) - (long long)(bpk * ci >> 20);
+        const long long bp = (long long)(ps[i - 1] - p0) - (long long)(bpk * cp >> 20);
         mf = (bb - bp) < mf ? (bb - bp) : mf;
     }
     s_a[threadIdx.x] = md;
@@ -799,59 +831,61 @@ __global__ __launch_bounds__(1024) void k_ef_scalars(const uint64_t *__restrict_
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out->bpk = bpk;
-        out->min_delta = s_a[0];
-        out->min_diff = s_b[0];
-        out->total_bits = pos[nb];
+        out[blockIdx.x].bpk = bpk;
+        out[blockIdx.x].min_delta = s_a[0];
+        out[blockIdx.x].min_diff = s_b[0];
+        out[blockIdx.x].total_bits = ps[nb] - p0;
     }
 }
 
-struct EfFill {
-    const uint64_t *cum, *pos;
-    uint64_t nb, bpk, lc, lp;
-    long long min_delta, min_diff;
-    unsigned long long *lower, *upper_c, *upper_p;
-    uint8_t *jump;  // bytes
-    uint64_t jump_bytes;
-    uint32_t *err;
-};
-
-// DoubleEF.hpp:204-281: lower bits of both sequences interleaved per bucket, one
-// upper bit per bucket in each upper-bits array.
-__global__ __launch_bounds__(256) void k_ef_fill(EfFill f) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > f.nb) return;
-    const uint64_t cdelta = i * (uint64_t)f.min_delta, bdelta = i * (uint64_t)f.min_diff;
-    const uint64_t cv = f.cum[i] - cdelta;
-    const uint64_t pval = (uint64_t)((long long)f.pos[i] - (long long)(f.bpk * f.cum[i] >> 20)) - bdelta;
-    const uint64_t start = i * (f.lc + f.lp);
-    if (f.lc) or_bits(f.lower, start, cv & ((1ull << f.lc) - 1), (uint32_t)f.lc);
-    if (f.lp) or_bits(f.lower, start + f.lc, pval & ((1ull << f.lp) - 1), (uint32_t)f.lp);
-    const uint64_t uc = (cv >> f.lc) + i, up = (pval >> f.lp) + i;
-    atomicOr(&f.upper_c[uc >> 6], 1ull << (uc & 63));
-    atomicOr(&f.upper_p[up >> 6], 1ull << (up & 63));
-    // jump table (DoubleEF.hpp:283-349): every q-th element records the position
-    // of its upper bit, as a 64-bit head per super block and 16-bit offsets.
-    if ((i & kEfQMaskDev) == 0) {
-        const uint64_t sb = i >> 14;
-        const uint64_t idx = sb * (kEfSuperSizeDev * 2);
-        const uint64_t head = i & ~(uint64_t)((1 << 14) - 1);
-        const uint64_t hc = f.cum[head] - head * (uint64_t)f.min_delta;
-        const uint64_t hp = (uint64_t)((long long)f.pos[head] - (long long)(f.bpk * f.cum[head] >> 20)) -
-                            head * (uint64_t)f.min_diff;
-        const uint64_t head_c = (hc >> f.lc) + head, head_p = (hp >> f.lp) + head;
-        if (i == head) {
-            if ((idx + 1) * 8 + 8 <= f.jump_bytes) {
-                *(uint64_t *)(f.jump + idx * 8) = head_c;
-                *(uint64_t *)(f.jump + idx * 8 + 8) = head_p;
+// One workgroup per set, DoubleEF.hpp:204-349: lower bits of both sequences
+// interleaved per bucket, one upper bit per bucket in each upper-bits array, and
+// the jump table (every q-th element: 64-bit head per super block, 16-bit
+// offsets), computed directly from each sampled element's upper-bit position.
+__global__ __launch_bounds__(256) void k_ef_fill(const SetEnc *__restrict__ sets, const EfScalars *__restrict__ sc,
+                                                 const uint64_t *__restrict__ acc, const uint64_t *__restrict__ pos,
+                                                 unsigned long long *__restrict__ lower, unsigned long long *__restrict__ upper_c,
+                                                 unsigned long long *__restrict__ upper_p, uint8_t *__restrict__ jump,
+                                                 uint32_t *__restrict__ err) {
+    const SetEnc e = sets[blockIdx.x];
+    const EfScalars f = sc[blockIdx.x];
+    const uint64_t *cum = acc + e.bbase, *ps = pos + e.bbase;
+    const uint64_t c0 = cum[0], p0 = ps[0];
+    unsigned long long *lw = lower + e.lower_word, *ucw = upper_c + e.uc_word, *upw = upper_p + e.up_word;
+    uint8_t *jb = jump + e.jump_word * 8;
+    const uint64_t jump_bytes = e.jump_words * 8;
+    const uint64_t lc = e.lc, lp = e.lp;
+    auto upper_c_pos = [&](uint64_t i) { return (((cum[i] - c0) - i * (uint64_t)f.min_delta) >> lc) + i; };
+    auto upper_p_pos = [&](uint64_t i) {
+        const uint64_t ci = cum[i] - c0;
+        const uint64_t pv = (uint64_t)((long long)(ps[i] - p0) - (long long)(f.bpk * ci >> 20)) - i * (uint64_t)f.min_diff;
+        return (pv >> lp) + i;
+    };
+    for (uint64_t i = threadIdx.x; i <= e.nb; i += blockDim.x) {
+        const uint64_t ci = cum[i] - c0;
+        const uint64_t cv = ci - i * (uint64_t)f.min_delta;
+        const uint64_t pval = (uint64_t)((long long)(ps[i] - p0) - (long long)(f.bpk * ci >> 20)) - i * (uint64_t)f.min_diff;
+        const uint64_t start = i * (lc + lp);
+        if (lc) or_bits(lw, start, cv & ((1ull << lc) - 1), (uint32_t)lc);
+        if (lp) or_bits(lw, start + lc, pval & ((1ull << lp) - 1), (uint32_t)lp);
+        const uint64_t uc = (cv >> lc) + i, up = (pval >> lp) + i;
+        atomicOr(&ucw[uc >> 6], 1ull << (uc & 63));
+        atomicOr(&upw[up >> 6], 1ull << (up & 63));
+        if ((i & kEfQMaskDev) == 0) {
+            const uint64_t idx = (i >> 14) * (kEfSuperSizeDev * 2);
+            const uint64_t head = i & ~(uint64_t)((1 << 14) - 1);
+            const uint64_t head_c = upper_c_pos(head), head_p = upper_p_pos(head);
+            if (i == head && (idx + 2) * 8 <= jump_bytes) {
+                *(uint64_t *)(jb + idx * 8) = head_c;
+                *(uint64_t *)(jb + idx * 8 + 8) = head_p;
             }
-        }
-        const uint64_t oc = uc - head_c, op = up - head_p;
-        if (oc >= 65536 || op >= 65536) atomicOr(f.err, 4u);
-        const uint64_t slot = (idx + 2) * 8 + 4 * ((i & ((1 << 14) - 1)) >> 8);
-        if (slot + 4 <= f.jump_bytes) {
-            *(uint16_t *)(f.jump + slot) = (uint16_t)oc;
-            *(uint16_t *)(f.jump + slot + 2) = (uint16_t)op;
+            const uint64_t oc = uc - head_c, op = up - head_p;
+            if (oc >= 65536 || op >= 65536) atomicOr(err, 4u);
+            const uint64_t slot = (idx + 2) * 8 + 4 * ((i & ((1 << 14) - 1)) >> 8);
+            if (slot + 4 <= jump_bytes) {
+                *(uint16_t *)(jb + slot) = (uint16_t)oc;
+                *(uint16_t *)(jb + slot + 2) = (uint16_t)op;
+            }
         }
     }
 }

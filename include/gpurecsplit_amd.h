@@ -99,6 +99,17 @@ int rs_build(rs_ctx *ctx, const uint64_t *keys, size_t n, uint32_t leaf_size, si
 int rs_build_device(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf_size,
                     size_t bucket_size, int rotate, rs_mphf **out);
 
+/* Batched construction: num_sets independent MPHFs in one set of launches.  Set i
+ * owns keys [set_offsets[i], set_offsets[i+1]) of d_keys (device memory;
+ * set_offsets is a host array of num_sets+1 increasing offsets, every set
+ * non-empty).  All sets share leaf_size, bucket_size and rotate; out receives
+ * num_sets handles (host-queryable, no device copy), stats the batch's
+ * measurements (may be NULL).  Each handle equals what rs_build_device returns
+ * for its set alone.  No reference counterpart (the reference builds one MPHF
+ * per constructor call, GPURecSplit.cuh:484-489). */
+int rs_build_batch(rs_ctx *ctx, const uint64_t *d_keys, const uint64_t *set_offsets, size_t num_sets,
+                   uint32_t leaf_size, size_t bucket_size, int rotate, rs_mphf **out, rs_build_stats *stats);
+
 /* Serialization in the reference's stream layout. */
 size_t rs_mphf_serialized_size(const rs_mphf *h);
 int rs_mphf_serialize(const rs_mphf *h, uint8_t *buf, size_t cap);

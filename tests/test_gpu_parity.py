@@ -127,3 +127,36 @@ def test_stats_and_evals(torch_cuda):
     ref = OracleMPHF.build(keys, 12, 2000, threads=8)
     assert sum(st["evals"].values()) == ref.evals
     assert st["num_nodes"] == ref.nseeds
+
+
+def test_batch_each_set_matches_oracle(torch_cuda):
+    """rs_build_batch: every set's bytes equal the oracle's build of that set alone."""
+    torch = torch_cuda
+    rng = np.random.default_rng(5)
+    sizes = [1, 2, 10, 123, 999, 5000, 20000, 1, 3000, 777]
+    keys = np.concatenate([splitmix_keys(1000 + i, s) for i, s in enumerate(sizes)])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12)
+    assert len(rss) == len(sizes)
+    for i, rs in enumerate(rss):
+        k = keys[off[i]:off[i + 1]]
+        assert rs.serialize() == OracleMPHF.build(k, 12, 100).serialize(), f"set {i} (n={sizes[i]})"
+    assert st["num_buckets"] == sum(max(1, -(-s // 100)) for s in sizes)
+
+
+def test_batch_many_uneven_sets(torch_cuda):
+    """512 sets with log-uniform sizes (the batched workload's shape, scaled down)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    sizes = np.exp(rng.uniform(np.log(100), np.log(20000), size=512)).astype(np.int64)
+    keys = splitmix_keys(3, int(sizes.sum()))
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12)
+    for i in range(0, 512, 37):
+        k = keys[off[i]:off[i + 1]]
+        assert rss[i].serialize() == OracleMPHF.build(k, 12, 100).serialize(), f"set {i}"
+    for i in range(512):
+        q = rss[i].query(keys[off[i]:off[i + 1]])
+        assert np.array_equal(np.sort(q), np.arange(sizes[i], dtype=np.uint64)), f"set {i}"
