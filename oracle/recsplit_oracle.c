@@ -204,19 +204,43 @@ static inline uint32_t higher_split(const orc_params *P, uint32_t m) {
     return ((uint16_t)(m / 2 + P->upper_aggr - 1) / P->upper_aggr) * P->upper_aggr;
 }
 
+/* Lane vectors (GCC vector extensions; element-wise C semantics): 16 candidate
+ * seeds per step as two groups of 8, hashed in 32-bit lanes. */
+typedef uint64_t u64x8 __attribute__((vector_size(64)));
+typedef uint32_t u32x8 __attribute__((vector_size(32)));
+typedef int32_t i32x8 __attribute__((vector_size(32)));
+typedef int64_t i64x8 __attribute__((vector_size(64)));
+
+static const u64x8 LANE_OFF0 = {0, 1, 2, 3, 4, 5, 6, 7};
+
+/* remix32 of the folded seed-key sums, low 16 bits, for 8 lanes */
+static inline u32x8 h16x8(u64x8 z) {
+    u32x8 t = __builtin_convertvector(z, u32x8) ^ __builtin_convertvector(z >> 32, u32x8);
+    t ^= t >> 16;
+    t *= 0x85ebca6bU;
+    t ^= t >> 13;
+    t *= 0xc2b2ae35U;
+    t ^= t >> 16;
+    return t & 0xFFFFu;
+}
+static inline u32x8 remap_x8(u64x8 z, uint32_t n) { return (h16x8(z) * n) >> 16; }
+static inline u64x8 bcast64(uint64_t v) { return (u64x8){v, v, v, v, v, v, v, v}; }
+
 /* higherLevel, SIMDRecSplit.hpp:250-317: lowest x >= 0 with
  * #{ remap(key + start_seed[level] + x, m) < split } == split. */
 uint64_t orc_search_higher(const uint64_t *keys, uint32_t m, uint32_t split, int level,
                            uint64_t *evals) {
     const uint64_t base = start_seed[level];
+    const u64x8 off1 = LANE_OFF0 + 8;
     for (uint64_t x = 0;; x += LANES) {
-        uint32_t cnt[LANES] = {0};
+        i32x8 c0 = {0}, c1 = {0};
         for (uint32_t i = 0; i < m; i++) {
-            const uint64_t k = keys[i] + base + x;
-            for (int j = 0; j < LANES; j++) cnt[j] += remap32(k + (uint64_t)j, m) < split;
+            const u64x8 k = bcast64(keys[i] + base + x);
+            c0 -= (i32x8)(remap_x8(k + LANE_OFF0, m) < split);
+            c1 -= (i32x8)(remap_x8(k + off1, m) < split);
         }
         for (int j = 0; j < LANES; j++)
-            if (cnt[j] == split) {
+            if ((uint32_t)(j < 8 ? c0[j] : c1[j - 8]) == split) {
                 if (evals) *evals += (x + j + 1) * (uint64_t)m;
                 return x + j;
             }
@@ -253,10 +277,6 @@ static inline uint32_t part_inc_hi(uint32_t part, uint32_t fanout) {
  * the uncounted part's target is <= 254): a carry out of any field lowers some
  * other count below zero.  part = floor(r / SPLIT) uses M = ceil(2^20 / SPLIT):
  * (r * M) >> 20 is exact for r < 3000 whenever SPLIT < 349. */
-static inline uint64_t part_inc64(uint32_t r, uint32_t magic) {
-    const uint32_t part = (r * magic) >> 20;
-    return part < 8 ? UINT64_C(1) << (8 * part) : 0;
-}
 static inline uint64_t aggr_target64(uint32_t m, uint32_t split, uint32_t fanout) {
     uint64_t T = 0;
     for (uint32_t p = 0; p < fanout && p < 8; p++)
@@ -265,22 +285,41 @@ static inline uint64_t aggr_target64(uint32_t m, uint32_t split, uint32_t fanout
 }
 static inline uint32_t split_magic(uint32_t split) { return ((1u << 20) + split - 1) / split; }
 
+/* The reference's lookup (SIMDRecSplit.hpp:144-148, 354, 378-379): increments of
+ * the low word (parts 0-3) and of the high word (parts 4-7) by part index; parts
+ * 8..15 add nothing (the uncounted last part of a fanout-9 node). */
+static const u32x8 INC_LO_A = {1u, 1u << 8, 1u << 16, 1u << 24, 0, 0, 0, 0}, INC_LO_B = {0};
+static const u32x8 INC_HI_A = {0, 0, 0, 0, 1u, 1u << 8, 1u << 16, 1u << 24}, INC_HI_B = {0};
+
 uint64_t orc_search_aggr(const uint64_t *keys, uint32_t m, uint32_t split, uint64_t seed,
                          uint64_t *evals) {
     const uint32_t fanout = (uint16_t)(m + split - 1) / split;
-    const uint64_t T = aggr_target64(m, split, fanout);
+    uint32_t tlo, thi;
+    aggr_targets(m, split, fanout, &tlo, &thi);
     const uint32_t magic = split_magic(split);
+    const u64x8 off1 = LANE_OFF0 + 8;
+    /* fanout <= 5: parts 0-3 in one word, part 4 uncounted (its lookup is 0) */
+    const int two_words = fanout > 5;
     for (uint64_t x = 0;; x += LANES) {
-        uint64_t cnt[LANES] = {0};
+        u32x8 lo0 = {0}, lo1 = {0}, hi0 = {0}, hi1 = {0};
         for (uint32_t i = 0; i < m; i++) {
-            const uint64_t k = keys[i] + seed + x;
-            for (int j = 0; j < LANES; j++) cnt[j] += part_inc64(remap32(k + (uint64_t)j, m), magic);
+            const u64x8 k = bcast64(keys[i] + seed + x);
+            const u32x8 p0 = (remap_x8(k + LANE_OFF0, m) * magic) >> 20;
+            const u32x8 p1 = (remap_x8(k + off1, m) * magic) >> 20;
+            lo0 += __builtin_shuffle(INC_LO_A, INC_LO_B, p0);
+            lo1 += __builtin_shuffle(INC_LO_A, INC_LO_B, p1);
+            if (two_words) {
+                hi0 += __builtin_shuffle(INC_HI_A, INC_HI_B, p0);
+                hi1 += __builtin_shuffle(INC_HI_A, INC_HI_B, p1);
+            }
         }
-        for (int j = 0; j < LANES; j++)
-            if (cnt[j] == T) {
+        for (int j = 0; j < LANES; j++) {
+            const uint32_t lo = j < 8 ? lo0[j] : lo1[j - 8], hi = j < 8 ? hi0[j] : hi1[j - 8];
+            if (lo == tlo && hi == thi) {
                 if (evals) *evals += (x + j + 1) * (uint64_t)m;
                 return x + j;
             }
+        }
     }
 }
 
@@ -317,22 +356,26 @@ uint64_t orc_search_leaf_rot(const uint64_t *keys, uint32_t L, uint64_t *evals) 
         if (keys[i] % 2 == 0) left[nl++] = keys[i];
         else right[nr++] = keys[i];
     }
+    const u64x8 step0 = LANE_OFF0 * L, step1 = (LANE_OFF0 + 8) * L;
     for (uint64_t b = 0;; b += (uint64_t)LANES * L) {
-        uint32_t ml[LANES] = {0}, mr[LANES] = {0};
+        u32x8 ml0 = {0}, ml1 = {0}, mr0 = {0}, mr1 = {0};
         for (int i = 0; i < nl; i++) {
-            const uint64_t k = left[i] + SEED + b;
-            for (int j = 0; j < LANES; j++) ml[j] |= 1u << remap32(k + (uint64_t)j * L, L);
+            const u64x8 k = bcast64(left[i] + SEED + b);
+            ml0 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + step0, L);
+            ml1 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + step1, L);
         }
         for (int i = 0; i < nr; i++) {
-            const uint64_t k = right[i] + SEED + b;
-            for (int j = 0; j < LANES; j++) mr[j] |= 1u << remap32(k + (uint64_t)j * L, L);
+            const u64x8 k = bcast64(right[i] + SEED + b);
+            mr0 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + step0, L);
+            mr1 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + step1, L);
         }
         for (int j = 0; j < LANES; j++) {
+            const uint32_t ml = j < 8 ? ml0[j] : ml1[j - 8], mr = j < 8 ? mr0[j] : mr1[j - 8];
             /* lossless prefilter: a cover needs both masks collision-free */
-            if (__builtin_popcount(ml[j]) != nl || __builtin_popcount(mr[j]) != nr) continue;
+            if (__builtin_popcount(ml) != nl || __builtin_popcount(mr) != nr) continue;
             for (uint32_t r = 0; r < L; r++) {
-                const uint32_t rot = r == 0 ? mr[j] : (((mr[j] << r) | (mr[j] >> (L - r))) & full);
-                if ((ml[j] | rot) == full) {
+                const uint32_t rot = r == 0 ? mr : (((mr << r) | (mr >> (L - r))) & full);
+                if ((ml | rot) == full) {
                     const uint64_t x = b + (uint64_t)j * L + r;
                     if (evals) *evals += (b / L + j + 1) * (uint64_t)L;
                     return x;
@@ -357,15 +400,17 @@ uint64_t orc_search_leaf_brute(const uint64_t *keys, uint32_t m, uint64_t *evals
     const uint64_t SEED = start_seed[NUM_START_SEEDS - 1];
     const uint32_t found = (uint32_t)((1ULL << m) - 1);
     const uint32_t midstop = m >= 12 ? bij_midstop(m) : m;
+    const u64x8 off1 = LANE_OFF0 + 8;
     for (uint64_t x = 0;; x += LANES) {
-        uint32_t mask[LANES] = {0};
+        u32x8 mk0 = {0}, mk1 = {0};
         for (uint32_t i = 0; i < midstop; i++) {
-            const uint64_t k = keys[i] + SEED + x;
-            for (int j = 0; j < LANES; j++) mask[j] |= 1u << remap32(k + (uint64_t)j, m);
+            const u64x8 k = bcast64(keys[i] + SEED + x);
+            mk0 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + LANE_OFF0, m);
+            mk1 |= (u32x8){1, 1, 1, 1, 1, 1, 1, 1} << remap_x8(k + off1, m);
         }
         for (int j = 0; j < LANES; j++) {
-            if ((uint32_t)__builtin_popcount(mask[j]) != midstop) continue;
-            uint32_t mk = mask[j];
+            uint32_t mk = j < 8 ? mk0[j] : mk1[j - 8];
+            if ((uint32_t)__builtin_popcount(mk) != midstop) continue;
             for (uint32_t i = midstop; i < m; i++) mk |= 1u << remap32(keys[i] + SEED + x + j, m);
             if (mk == found) {
                 if (evals) *evals += (x + j + 1) * (uint64_t)m;
