@@ -23,15 +23,23 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# MI355X int32 VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12
-# full-rate lane operations per second (the published 157.3 TFLOP/s FP32 vector
-# figure counts packed FMA: x2 flops, x2 packed).
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# Roofline of the seed search: int32 VALU issue.  Peak = MI355X's published FP32
+# vector peak (157.3 TFLOP/s) as lane operations: one lane-op per FMA.
+VALU_PEAK_TOPS = 157.3 / 2
+# VALU instructions per key-seed evaluation in the inner loop of each search
+# kernel (default launch: u=4, LDS increment table), counted from the gfx950
+# assembly of librecsplit_amd.so, and the same weighted by the issue cost measured
+# by rs_probe.hip (v_mul_lo_u32 1.72, v_lshl_add_u64 1.78, v_mul_u32_u24 1.61
+# v_xor_b32 slots).
+INSTR_PER_EVAL = {"fanout2": 11, "upper_aggr": 12.5, "lower_aggr": 12.5, "leaf_rotation": 12, "leaf_bruteforce": 12}
+SLOTS_PER_EVAL = {"fanout2": 13.2, "upper_aggr": 15.3, "lower_aggr": 15.3, "leaf_rotation": 14.8,
+                  "leaf_bruteforce": 14.8}
+XOR_RATE = 5.747e13  # v_xor_b32 lane-ops/s measured by rs_probe.hip
 
-# VALU issue cycles per key-seed evaluation of each search kernel's inner loop,
-# counted from the gfx950 assembly of librecsplit_amd.so (full-rate op = 1,
-# v_mul_lo_u32 = 4).  Index: fanout-2, upper aggr, lower aggr, rotation leaf, brute leaf.
-OPS_PER_EVAL = {"fanout2": 17, "upper_aggr": 20, "lower_aggr": 20, "leaf_rotation": 22, "leaf_bruteforce": 19}
+# HBM bytes per launch of each search kernel at this workload, from two rocprofv3
+# --pmc runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, the factors of the 256 MiB copy
+# probe); profiles/k_search_plain_l16_b2000_n5M.md.
+TRAFFIC_BYTES = {"lower_aggr": 16.60e9, "upper_aggr": 4.55e9, "leaf_rotation": 10.66e9}
 
 GAMMA = np.uint64(0x9E3779B97F4A7C15)
 
@@ -82,6 +90,47 @@ def probe(device: int = 0) -> dict:
             "copy_256MiB_ms": ms.value, "copy_GBps": 2 * (256 << 20) / (ms.value / 1e3) / 1e9}
 
 
+def batched(gra, ctx, args, plain_ms: float) -> dict:
+    """B sets of n keys each (splitmix64 seeds seed..seed+B-1) built in one batch."""
+    import torch
+
+    B = args.batch_sets
+    keys = np.concatenate([splitmix64_keys(args.seed + i, args.n) for i in range(B)])
+    off = np.arange(B + 1, dtype=np.uint64) * np.uint64(args.n)
+    d = torch.from_numpy(keys.view(np.int64)).to("cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rss, st = gra.build_batch(d, off, args.bucket, leaf_size=args.leaf, ctx=ctx)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    # the first set is the plain workload's key set: identical output
+    same = rss[0].serialize() == gra.GPURecSplit(d[: args.n], args.bucket, leaf_size=args.leaf, ctx=ctx).serialize()
+    return {"sets": B, "keys_per_set": args.n, "ms": ms, "value": ms * 1e6 / (B * args.n), "unit": "ns/object",
+            "objects_per_s_vs_plain": (B * args.n / ms) / (args.n / plain_ms), "first_set_equals_plain": same}
+
+
+def served(args) -> dict:
+    """Served stream (scaled): Poisson arrivals, log-uniform sizes in [1e4, 1e6],
+    l in {8, 12, 16, 24} with weights {.3, .3, .3, .1}, b = 2000 for l >= 16 else 100;
+    l = 24 jobs capped at 5000 keys (an l=24 key costs ~3e8 evaluations)."""
+    from gpurecsplit_amd.serve import poisson_stream, run_stream
+
+    stream = poisson_stream(args.serve_jobs, rate_per_s=6.0, seed=4, size_lo=10_000, size_hi=1_000_000,
+                            leaf_weights={8: 0.3, 12: 0.3, 16: 0.3, 24: 0.1},
+                            bucket_of_leaf=lambda l: 2000 if l >= 16 else 100,
+                            size_cap_of_leaf=lambda l: 5000 if l == 24 else 10**9)
+    jobs, summ = run_stream(stream, workers=2)
+    errors = [j.error for j in jobs if j.error]
+    ok = not errors
+    for j in jobs[:: max(1, len(jobs) // 10)]:  # bijection of a sample of results
+        if j.result is not None:
+            q = j.result.query(j.keys)
+            ok &= bool(np.array_equal(np.sort(q), np.arange(j.n, dtype=np.uint64)))
+    summ.update({"stream": "scaled: rate 6/s, sizes log-uniform [1e4, 1e6], l=24 capped at 5000 keys, 2 workers",
+                 "verified_sample": ok, "errors": errors[:3]})
+    return summ
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +142,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", action="store_true", help="also run the VALU issue-rate and copy probes")
+    ap.add_argument("--batch-sets", type=int, default=4, help="sets of n keys built together (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=120, help="jobs of the served stream (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -137,7 +188,8 @@ def main() -> int:
     kind = max(last["search_ms"], key=lambda k: last["search_ms"][k])
     kms = np.mean([s["search_ms"][kind] for s in stats])
     evals = last["evals"][kind]
-    achieved = evals * OPS_PER_EVAL[kind] / (kms / 1e3) / 1e12
+    evals_per_s = evals / (kms / 1e3)
+    achieved = evals_per_s * INSTR_PER_EVAL[kind] / 1e12
     total_evals = sum(last["evals"].values())
     search_total_ms = sum(last["search_ms"].values())
 
@@ -159,8 +211,9 @@ def main() -> int:
                    "leaf": args.leaf, "bucket": args.bucket, "n": args.n},
         "roofline": {"bound": "valu", "kernel": f"k_search<{kind}>", "achieved": achieved,
                      "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
-                     "traffic": None, "evals": evals, "kernel_ms": kms,
-                     "ops_per_eval": OPS_PER_EVAL[kind]},
+                     "traffic": TRAFFIC_BYTES.get(kind), "evals": evals, "kernel_ms": kms,
+                     "evals_per_s": evals_per_s, "instr_per_eval": INSTR_PER_EVAL[kind],
+                     "issue_bound_frac": evals_per_s * SLOTS_PER_EVAL[kind] / XOR_RATE},
         "seeds_evals_per_s": total_evals / (search_total_ms / 1e3),
         "exec_ratio": {k: (last["executed_evals"][k] / last["evals"][k]) if last["evals"][k] else None
                        for k in last["evals"]},
@@ -169,6 +222,10 @@ def main() -> int:
         "phase_ms": {"partition": last["partition_ms"], "search": last["search_ms"], "split": last["split_ms"],
                      "encode": last["encode_ms"], "total": last["total_ms"]},
     }
+    if args.batch_sets > 1:
+        line["batched"] = batched(gra, ctx, args, ms_per_step)
+    if args.serve_jobs > 0:
+        line["served"] = served(args)
     if args.probe:
         line["probe"] = probe(0)
     if not args.no_cpu_baseline:

@@ -43,11 +43,14 @@ struct NodeDesc {
     uint8_t kind;
 };
 
-// Search state of one node (16 B): lowest valid seed found so far and the next
-// unclaimed chunk of candidate seeds.
+// Search state of one node (32 B): lowest valid seed found so far, the next
+// unclaimed chunk of candidate seeds, and the number of waves working on it.
 struct NodeState {
     unsigned long long result;
     unsigned long long next_chunk;
+    unsigned int waves;
+    unsigned int pad0;
+    unsigned long long pad1;
 };
 
 // Split parameters of a leaf size (AbstractParallelRecSplit.hpp:133-148).

@@ -255,28 +255,37 @@ struct SearchArgs {
     uint32_t lut_stride;  // LDS words per wave for the aggregation increment table
 };
 
-// Streams keys kp[begin, end) through scalar registers; the next four keys are
-// loaded while the current four are hashed.  The key array carries 8 entries of
-// padding, so the look-ahead loads (up to kp[end + 3]) stay inside it.
+// Opaque per-lane zero: indexes derived from it are not provably uniform, so key
+// loads become vector loads (vmcnt, L1/L2) instead of scalar loads, whose lgkmcnt
+// waits would also block on the LDS table reads of the aggregation search.
+__device__ __forceinline__ uint32_t lane_zero() {
+    uint32_t z;
+    __asm__("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+// Streams keys kp[begin, end) (the same for every lane) into registers; the next
+// four keys are loaded while the current four are hashed.  The key array carries
+// 8 entries of padding, so the look-ahead loads (up to kp[end + 3]) stay inside it.
 template <typename F>
-__device__ __forceinline__ void stream_keys(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end,
-                                            uint64_t sconst, F &&f) {
+__device__ __forceinline__ void stream_keys(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end, F &&f) {
+    const uint64_t *p = kp + lane_zero();
     uint32_t i = begin;
-    uint64_t n0 = kp[i], n1 = kp[i + 1], n2 = kp[i + 2], n3 = kp[i + 3];
+    uint64_t n0 = p[i], n1 = p[i + 1], n2 = p[i + 2], n3 = p[i + 3];
     for (; i + 4 <= end; i += 4) {
-        const uint64_t c0 = n0 + sconst, c1 = n1 + sconst, c2 = n2 + sconst, c3 = n3 + sconst;
-        n0 = kp[i + 4];
-        n1 = kp[i + 5];
-        n2 = kp[i + 6];
-        n3 = kp[i + 7];
+        const uint64_t c0 = n0, c1 = n1, c2 = n2, c3 = n3;
+        n0 = p[i + 4];
+        n1 = p[i + 5];
+        n2 = p[i + 6];
+        n3 = p[i + 7];
         f(c0);
         f(c1);
         f(c2);
         f(c3);
     }
-    if (i < end) f(n0 + sconst);
-    if (i + 1 < end) f(n1 + sconst);
-    if (i + 2 < end) f(n2 + sconst);
+    if (i < end) f(n0);
+    if (i + 1 < end) f(n1);
+    if (i + 2 < end) f(n2);
 }
 
 // Fanout-2 node: lowest x with #{remap(k+S[depth]+x, m) < split} == split
@@ -289,10 +298,10 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
     uint32_t cnt[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        s[u] = u0 + (uint64_t)(u * 64 + lane) + sconst;
         cnt[u] = 0;
     }
-    stream_keys(kp, 0, m, sconst, [&](uint64_t k) {
+    stream_keys(kp, 0, m, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) cnt[u] += fold_h16(k + s[u]) < H;
     });
@@ -320,10 +329,10 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
     C cnt[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        s[u] = u0 + (uint64_t)(u * 64 + lane) + sconst;
         cnt[u] = 0;
     }
-    stream_keys(kp, 0, m, sconst, [&](uint64_t k) {
+    stream_keys(kp, 0, m, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t r = __umul24(fold_h16(k + s[u]), m) >> 16;
@@ -357,14 +366,14 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
     uint32_t ml[U], mr[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        sb[u] = (u0 + (uint64_t)(u * 64 + lane)) * L;
+        sb[u] = (u0 + (uint64_t)(u * 64 + lane)) * L + S;
         ml[u] = mr[u] = 0;
     }
-    stream_keys(kp, 0, nl, S, [&](uint64_t k) {
+    stream_keys(kp, 0, nl, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) ml[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
     });
-    stream_keys(kp, nl, L, S, [&](uint64_t k) {
+    stream_keys(kp, nl, L, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) mr[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
     });
@@ -405,10 +414,10 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
     uint32_t mk[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane);
+        s[u] = u0 + (uint64_t)(u * 64 + lane) + S;
         mk[u] = 0;
     }
-    stream_keys(kp, 0, mid, S, [&](uint64_t k) {
+    stream_keys(kp, 0, mid, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
     });
@@ -417,7 +426,7 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
 #pragma unroll
         for (int u = 0; u < U; u++) any |= (uint32_t)__popc(mk[u]) == mid;
         if (!__ballot(any)) return kNone;
-        stream_keys(kp, mid, m, S, [&](uint64_t k) {
+        stream_keys(kp, mid, m, [&](uint64_t k) {
 #pragma unroll
             for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
         });
@@ -463,6 +472,9 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             }
             if (helping) {
                 int found = -1;
+                // Among the first 64 running entries with work left, join the node
+                // with the fewest waves (spreads helpers evenly over the nodes
+                // still open; a node's chunk overshoot grows with its waves).
                 for (uint32_t k = 0; k < a.n_waves && found < 0; k += 64) {
                     uint32_t idx = scan + k + lane;
                     while (idx >= a.n_waves) idx -= a.n_waves;
@@ -471,7 +483,14 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                     const bool ok = cand >= 0 && has_work(&a.state[cand], units_per_chunk, scale);
                     const unsigned long long msk = __ballot(ok);
                     if (msk) {
-                        const int l = __ffsll((long long)msk) - 1;
+                        uint32_t key = 0xFFFFFFFFu;
+                        if (ok) {
+                            const uint32_t w = __hip_atomic_load(&a.state[cand].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            key = (min(w, 0xFFFFu) << 16) | lane;
+                        }
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o));
+                        const int l = (int)(uni(key) & 0xFFFF);
                         found = (int)uni((uint32_t)__shfl(cand, l));
                         scan = uni(__shfl(idx, l));
                     }
@@ -480,7 +499,10 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 node = found;
             }
             node = (int)uni((uint32_t)node);
-            if (lane == 0) __hip_atomic_store(&a.running[wave], node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) {
+                __hip_atomic_store(&a.running[wave], node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(&a.state[node].waves, 1u);
+            }
             const NodeDesc d = a.nodes[node];
             key_off = uni64(d.key_off);
             m = uni(d.m);
@@ -526,8 +548,11 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
         const uint64_t unit0 = chunk * units_per_chunk;
         if (unit0 * scale >= res || unit0 * scale >= kSeedLimit) {
             if (unit0 * scale >= kSeedLimit && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)kSeedLimit);
+            if (lane == 0) {
+                __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicSub(&a.state[node].waves, 1u);
+            }
             node = -1;
-            if (lane == 0) __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             continue;
         }
         const uint64_t *kp = keys + key_off;
@@ -561,6 +586,7 @@ __global__ __launch_bounds__(256) void k_state_init(NodeState *st, uint32_t n) {
     if (i < n) {
         st[i].result = kNone;
         st[i].next_chunk = 0;
+        st[i].waves = 0;
     }
 }
 
