@@ -95,6 +95,7 @@ def batched(gra, ctx, args, plain_ms: float) -> dict:
     import torch
 
     B = args.batch_sets
+    print(f"# batched: {B} sets of {args.n} keys", file=sys.stderr, flush=True)
     keys = np.concatenate([splitmix64_keys(args.seed + i, args.n) for i in range(B)])
     off = np.arange(B + 1, dtype=np.uint64) * np.uint64(args.n)
     d = torch.from_numpy(keys.view(np.int64)).to("cuda:0")
@@ -103,6 +104,7 @@ def batched(gra, ctx, args, plain_ms: float) -> dict:
     rss, st = gra.build_batch(d, off, args.bucket, leaf_size=args.leaf, ctx=ctx)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
+    print(f"# batched: {ms:.1f} ms", file=sys.stderr, flush=True)
     # the first set is the plain workload's key set: identical output
     same = rss[0].serialize() == gra.GPURecSplit(d[: args.n], args.bucket, leaf_size=args.leaf, ctx=ctx).serialize()
     return {"sets": B, "keys_per_set": args.n, "ms": ms, "value": ms * 1e6 / (B * args.n), "unit": "ns/object",
@@ -119,7 +121,16 @@ def served(args) -> dict:
                             leaf_weights={8: 0.3, 12: 0.3, 16: 0.3, 24: 0.1},
                             bucket_of_leaf=lambda l: 2000 if l >= 16 else 100,
                             size_cap_of_leaf=lambda l: 5000 if l == 24 else 10**9)
-    jobs, summ = run_stream(stream, workers=2)
+    print(f"# served: {len(stream)} jobs", file=sys.stderr, flush=True)
+    done = [0]
+
+    def on_batch(batch):
+        done[0] += len(batch)
+        print(f"# served: batch of {len(batch)} (l={batch[0].leaf}, {sum(j.n for j in batch)} keys) done, "
+              f"{done[0]}/{len(stream)} jobs", file=sys.stderr, flush=True)
+
+    jobs, summ = run_stream(stream, workers=2, time_limit_s=args.serve_time_limit, on_batch=on_batch)
+    print(f"# served: done in {summ['wall_s']:.1f} s", file=sys.stderr, flush=True)
     errors = [j.error for j in jobs if j.error]
     ok = not errors
     for j in jobs[:: max(1, len(jobs) // 10)]:  # bijection of a sample of results
@@ -143,7 +154,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", action="store_true", help="also run the VALU issue-rate and copy probes")
     ap.add_argument("--batch-sets", type=int, default=4, help="sets of n keys built together (0: skip)")
-    ap.add_argument("--serve-jobs", type=int, default=120, help="jobs of the served stream (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=0,
+                    help="jobs of the served stream (0: skip; opt-in while the l=24 stream is under investigation)")
+    ap.add_argument("--serve-time-limit", type=float, default=60.0, help="per-batch build time limit of the served stream (s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

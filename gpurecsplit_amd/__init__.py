@@ -28,7 +28,7 @@ LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "bucket with 3000 or more keys",
     4: "duplicate keys", 5: "seed limit exceeded", 6: "Elias-Fano offset overflow", 7: "out of memory",
-    8: "malformed serialized stream", 9: "no device copy",
+    8: "malformed serialized stream", 9: "no device copy", 10: "time limit exceeded",
 }
 
 KIND_NAMES = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforce"]
@@ -78,6 +78,7 @@ def _load():
     sig = {
         "rs_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "rs_ctx_destroy": (None, [vp]),
+        "rs_ctx_set_time_limit": (C.c_int, [vp, C.c_double]),
         "rs_build": (C.c_int, [vp, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.POINTER(vp)]),
         "rs_build_device": (C.c_int, [vp, C.c_void_p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.POINTER(vp)]),
         "rs_mphf_serialized_size": (C.c_size_t, [vp]),
@@ -135,11 +136,18 @@ class Context:
 
     _default: dict[int, "Context"] = {}
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, time_limit_s: float | None = None):
         h = C.c_void_p()
         _check(lib.rs_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        if time_limit_s is not None:
+            self.set_time_limit(time_limit_s)
+
+    def set_time_limit(self, seconds: float):
+        """Bounds every later build on this context (0: no limit); a build past
+        the limit raises RecSplitError with code 10 and the context stays usable."""
+        _check(lib.rs_ctx_set_time_limit(self.h, float(seconds)))
 
     @classmethod
     def default(cls, device: int = 0) -> "Context":

@@ -226,7 +226,9 @@ struct rs_ctx {
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
         list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
-        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off;
+        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock;
+    double time_limit_s = 0;  // 0: none
+    int wall_khz = 0;         // device wall-clock rate
     uint32_t *memo[kMaxLeaf + 1] = {};
     uint32_t *h_pinned = nullptr;  // small pinned staging area
     hipEvent_t ev[128];
@@ -384,6 +386,15 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     const uint64_t n_total = set_off[S] - set_off[0];
     st.num_buckets = NB;
 
+    // ---- 0. time limit reference point
+    unsigned long long *clock0 = nullptr, limit_ticks = 0;
+    if (c->time_limit_s > 0 && c->wall_khz > 0) {
+        clock0 = c->clock.get<unsigned long long>(1);
+        limit_ticks = std::max<unsigned long long>(1, (unsigned long long)(c->time_limit_s * c->wall_khz * 1e3));
+        k_clock_mark<<<1, 64, 0, s>>>(clock0);
+        HIP_CHECK(hipGetLastError());
+    }
+
     // ---- 1. bucket histogram and per-bucket metadata
     const int e_part0 = ev_record(c);
     KeyTile *d_tiles = c->tiles.get<KeyTile>(tiles.size());
@@ -494,6 +505,9 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
         a.leaf = P.leaf;
         a.upper = P.upper;
         a.work = work + l;
+        a.clock0 = clock0;
+        a.limit_ticks = limit_ticks;
+        a.err = err;
         a.groups = (uint32_t)tn.groups[kind];
         bool wide = false;
         size_t lds = 0;
@@ -545,6 +559,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
     for (int l = 0; l < kNumLists; l++)
         st.executed_evals[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += h64[16 + l];
+    if (h[16] & 8) throw BuildError(RS_ERR_TIMEOUT, "the build passed the context's time limit during the seed search");
     if (h[16] & 2) throw BuildError(RS_ERR_SEED_OVERFLOW, "a node search passed the 2^44 seed limit");
 
     // ---- 6. Golomb-Rice: per-set vectors packed one after another
@@ -736,6 +751,8 @@ int rs_ctx_create(int device, rs_ctx **out) {
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount;
         c->tune = parse_tune(getenv("RS_TUNE"));
+        HIP_CHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
+        if (const char *tl = getenv("RS_TIME_LIMIT")) c->time_limit_s = std::max(0.0, atof(tl));
         HIP_CHECK(hipStreamCreate(&c->stream));  // blocking: ordered after null-stream work
         HIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * 1024, hipHostMallocDefault));
         for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
@@ -748,6 +765,16 @@ int rs_ctx_create(int device, rs_ctx **out) {
     return RS_OK;
 }
 
+int rs_ctx_set_time_limit(rs_ctx *c, double seconds) {
+    if (!c || !(seconds >= 0)) {
+        set_error("rs_ctx_set_time_limit: null context or negative limit");
+        return RS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(c->mu);
+    c->time_limit_s = seconds;
+    return RS_OK;
+}
+
 void rs_ctx_destroy(rs_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -756,7 +783,7 @@ void rs_ctx_destroy(rs_ctx *c) {
                       &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
                       &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
                       &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
-                      &c->set_gather, &c->set_enc, &c->bit_off})
+                      &c->set_gather, &c->set_enc, &c->bit_off, &c->clock})
         b->release();
     for (auto &m : c->memo)
         if (m) (void)hipFree(m);

@@ -253,7 +253,18 @@ struct SearchArgs {
     uint32_t magic;       // ceil(2^20 / split_unit)
     uint32_t upper;       // upper aggregation bound (fanout-2 split size)
     uint32_t lut_stride;  // LDS words per wave for the aggregation increment table
+    // Time limit: 0 = none; otherwise waves leave the search once the device wall
+    // clock passes *clock0 + limit_ticks, setting bit 3 of *err.
+    const unsigned long long *clock0;
+    unsigned long long limit_ticks;
+    uint32_t *err;
 };
+
+// Build start on the device wall clock (100 MHz constant clock), the reference
+// point of the time limit.
+__global__ void k_clock_mark(unsigned long long *t0) {
+    if (threadIdx.x == 0) *t0 = (unsigned long long)wall_clock64();
+}
 
 // Opaque per-lane zero: indexes derived from it are not provably uniform, so key
 // loads become vector loads (vmcnt, L1/L2) instead of scalar loads, whose lgkmcnt
@@ -460,6 +471,7 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
     // cached per-node values (wave-uniform)
     uint64_t key_off = 0, sconst = 0, T = 0, cmask = 0;
     uint32_t m = 0, aux0 = 0, aux1 = 0;
+    const unsigned long long t_end = a.limit_ticks ? uni64(*a.clock0) + a.limit_ticks : ~0ull;
 
     for (;;) {
         if (node < 0) {
@@ -536,6 +548,20 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                     if (mid > m) mid = m;
                 }
                 aux0 = mid;
+            }
+        }
+        if (a.limit_ticks) {
+            // past the time limit: every wave leaves at its next claim, so the
+            // launch drains within one chunk of work per wave
+            uint32_t late = 0;
+            if (lane == 0) late = (unsigned long long)wall_clock64() > t_end;
+            if (uni(late)) {
+                if (lane == 0) {
+                    atomicOr(a.err, 8u);
+                    __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicSub(&a.state[node].waves, 1u);
+                }
+                break;
             }
         }
         unsigned long long chunk = 0, res = 0;

@@ -65,12 +65,15 @@ class Server:
     """Continuous-batching construction server over `workers` concurrent contexts."""
 
     def __init__(self, workers: int = 2, hbm_budget_bytes: int = 64 << 30, max_batch_keys: int = 50_000_000,
-                 device: int = 0):
+                 device: int = 0, time_limit_s: float | None = None, on_batch=None):
+        """time_limit_s bounds each batch build (its jobs fail with the time-limit
+        error past it); on_batch(jobs) is called after every finished batch."""
         import torch
 
         self.torch = torch
         self.device = device
-        self.ctxs = [Context(device) for _ in range(workers)]
+        self.on_batch = on_batch
+        self.ctxs = [Context(device, time_limit_s=time_limit_s) for _ in range(workers)]
         self.budget = hbm_budget_bytes
         self.max_batch_keys = max_batch_keys
         self.pending: list[Job] = []
@@ -156,6 +159,8 @@ class Server:
                 self.stats.batches += 1
                 self.admitted_bytes -= sum(j.n * BYTES_PER_KEY for j in batch)
                 self.cv.notify_all()
+            if self.on_batch is not None:
+                self.on_batch(batch)
 
 
 def poisson_stream(num_jobs: int, rate_per_s: float, seed: int, size_lo: int, size_hi: int,
@@ -186,11 +191,12 @@ def splitmix64_keys(seed: int, n: int) -> np.ndarray:
     return z.reshape(n, 2)
 
 
-def run_stream(stream, workers: int = 2, hbm_budget_bytes: int = 64 << 30) -> tuple[list[Job], dict]:
+def run_stream(stream, workers: int = 2, hbm_budget_bytes: int = 64 << 30, time_limit_s: float | None = None,
+               on_batch=None) -> tuple[list[Job], dict]:
     """Replays a stream in real time against a Server and returns (jobs, summary).
     Keys are generated before the server clock starts."""
     prepared = [(t_arr, splitmix64_keys(kseed, n), leaf, bucket) for (t_arr, n, leaf, bucket, kseed) in stream]
-    srv = Server(workers=workers, hbm_budget_bytes=hbm_budget_bytes)
+    srv = Server(workers=workers, hbm_budget_bytes=hbm_budget_bytes, time_limit_s=time_limit_s, on_batch=on_batch)
     jobs = []
     for i, (t_arr, keys, leaf, bucket) in enumerate(prepared):
         delay = t_arr - srv.now()

@@ -63,7 +63,8 @@ enum rs_status {
     RS_ERR_EF_OVERFLOW = 6,       /* Elias-Fano jump offset >= 2^16 (reference aborts) */
     RS_ERR_NOMEM = 7,             /* host or device allocation failed */
     RS_ERR_FORMAT = 8,            /* malformed serialized stream or leaf-size mismatch */
-    RS_ERR_NO_DEVICE = 9          /* handle has no device copy (deserialized) */
+    RS_ERR_NO_DEVICE = 9,         /* handle has no device copy (deserialized) */
+    RS_ERR_TIMEOUT = 10           /* the build passed the context's time limit */
 };
 
 /* Per-build measurements (HIP events on the build stream). */
@@ -88,6 +89,14 @@ typedef struct rs_build_stats {
 /* Context bound to one GPU (device ordinal as seen by HIP). */
 int rs_ctx_create(int device, rs_ctx **out);
 void rs_ctx_destroy(rs_ctx *ctx);
+
+/* Wall-clock limit for each later build on ctx, in seconds (0: none, the
+ * default; the environment variable RS_TIME_LIMIT sets it at creation).  Past
+ * the limit the seed-search kernels stop at their next work claim and the build
+ * returns RS_ERR_TIMEOUT; the context stays usable.  No reference counterpart
+ * (the reference search runs until every seed is found, GPURecSplit.cuh:508-577);
+ * it is for servers that must bound a job's time. */
+int rs_ctx_set_time_limit(rs_ctx *ctx, double seconds);
 
 /* Construction from host keys (n pairs first,second).  leaf_size in [2, 24],
  * bucket_size >= 1, rotate != 0 selects rotation fitting for full leaves

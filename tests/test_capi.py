@@ -71,3 +71,8 @@ def test_bits_count_matches_reference_formula():
     rs = gra.GPURecSplit.deserialize(data, 8)
     bits = rs.get_bits()
     assert 1.4 * 10000 < bits < 3 * 10000
+
+
+def test_time_limit_rejects_null_context():
+    assert gra.lib.rs_ctx_set_time_limit(None, 1.0) == 1
+    assert gra.lib.rs_ctx_set_time_limit(None, -1.0) == 1

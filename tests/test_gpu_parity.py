@@ -160,3 +160,18 @@ def test_batch_many_uneven_sets(torch_cuda):
     for i in range(512):
         q = rss[i].query(keys[off[i]:off[i + 1]])
         assert np.array_equal(np.sort(q), np.arange(sizes[i], dtype=np.uint64)), f"set {i}"
+
+
+def test_time_limit_stops_build_and_context_recovers(torch_cuda):
+    """A 1 us limit ends every search wave at its first claim (RS_ERR_TIMEOUT = 10);
+    with the limit lifted the same context builds the oracle's bytes."""
+    keys = splitmix_keys(15, 20_000)
+    ctx = gra.Context(0, time_limit_s=1e-6)
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx)
+    assert e.value.code == 10
+    ctx.set_time_limit(0)
+    rs = gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx)
+    assert rs.serialize() == OracleMPHF.build(keys, 12, 100).serialize()
+    ctx.set_time_limit(600)  # a generous limit changes nothing
+    assert gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx).serialize() == rs.serialize()
