@@ -27,31 +27,20 @@ sys.path.insert(0, str(ROOT))
 # vector peak (157.3 TFLOP/s) as lane operations: one lane-op per FMA.
 VALU_PEAK_TOPS = 157.3 / 2
 # VALU instructions per key-seed evaluation in the inner loop of each search
-# kernel (default launch: u=4, LDS increment table), counted from the gfx950
-# assembly of librecsplit_amd.so, and the same weighted by the issue cost measured
-# by rs_probe.hip (v_mul_lo_u32 1.72, v_lshl_add_u64 1.78, v_mul_u32_u24 1.61
-# v_xor_b32 slots).
-INSTR_PER_EVAL = {"fanout2": 11, "upper_aggr": 12.5, "lower_aggr": 12.5, "leaf_rotation": 12, "leaf_bruteforce": 12}
-SLOTS_PER_EVAL = {"fanout2": 13.2, "upper_aggr": 15.3, "lower_aggr": 15.3, "leaf_rotation": 14.8,
-                  "leaf_bruteforce": 14.8}
+# kernel (default launch: 4 candidates per lane, 1 for fanout-2 nodes; LDS
+# increment table), counted from the gfx950 assembly of librecsplit_amd.so, and
+# the same weighted by the issue cost measured by rs_probe.hip (v_mul_lo_u32 1.72,
+# v_lshl_add_u64 1.78, v_mul_u32_u24 / v_mad_u32_u24 1.61 v_xor_b32 slots).
+INSTR_PER_EVAL = {"fanout2": 11.5, "upper_aggr": 10.88, "lower_aggr": 10.88, "leaf_rotation": 10.94,
+                  "leaf_bruteforce": 10.88}
+SLOTS_PER_EVAL = {"fanout2": 13.91, "upper_aggr": 13.75, "lower_aggr": 13.75, "leaf_rotation": 13.86,
+                  "leaf_bruteforce": 13.80}
 XOR_RATE = 5.747e13  # v_xor_b32 lane-ops/s measured by rs_probe.hip
 
 # HBM bytes per launch of each search kernel at this workload, from two rocprofv3
 # --pmc runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, the factors of the 256 MiB copy
 # probe); profiles/k_search_plain_l16_b2000_n5M.md.
 TRAFFIC_BYTES = {"lower_aggr": 16.60e9, "upper_aggr": 4.55e9, "leaf_rotation": 10.66e9}
-
-GAMMA = np.uint64(0x9E3779B97F4A7C15)
-
-
-def splitmix64_keys(seed: int, n: int) -> np.ndarray:
-    """(n, 2) uint64 keys: first = next(), second = next() of SplitMix64(seed)."""
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed) + GAMMA * np.arange(1, 2 * n + 1, dtype=np.uint64)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return z.reshape(n, 2)
 
 
 def cpu_baseline(leaf: int, bucket: int, seed: int) -> dict:
@@ -79,7 +68,8 @@ def probe(device: int = 0) -> dict:
 
     lib = C.CDLL(str(ROOT / "gpurecsplit_amd" / "librs_probe.so"))
     rates = {}
-    for kind, name in enumerate(["v_xor_b32", "v_mul_lo_u32", "v_lshl_add_u64", "v_mul_u32_u24"]):
+    for kind, name in enumerate(["v_xor_b32", "v_mul_lo_u32", "v_lshl_add_u64", "v_mul_u32_u24", "v_xor_b32_sdwa",
+                                 "v_mad_u32_u24", "v_lshlrev_b32_sdwa"]):
         r, ms = C.c_double(), C.c_double()
         assert lib.rs_probe_valu(device, kind, 256, C.byref(r), C.byref(ms)) == 0
         rates[name] = r.value
@@ -93,6 +83,7 @@ def probe(device: int = 0) -> dict:
 def batched(gra, ctx, args, plain_ms: float) -> dict:
     """B sets of n keys each (splitmix64 seeds seed..seed+B-1) built in one batch."""
     import torch
+    from gpurecsplit_amd.keys import splitmix64_keys
 
     B = args.batch_sets
     print(f"# batched: {B} sets of {args.n} keys", file=sys.stderr, flush=True)
@@ -142,6 +133,33 @@ def served(args) -> dict:
     return summ
 
 
+def batched_4096(gra, ctx) -> dict:
+    """The 4096-set batched workload: per-set sizes log-uniform in [1e3, 1e6]
+    (numpy default_rng(3)), keys from one SplitMix64(3) stream generated in HBM,
+    l = 12, b = 100, all sets in one rs_build_batch call.  Verifies the bijection
+    of 32 evenly spaced sets by host queries."""
+    import torch
+    from gpurecsplit_amd.keys import splitmix64_keys_device
+
+    rng = np.random.default_rng(3)
+    sizes = np.exp(rng.uniform(np.log(1_000), np.log(1_000_000), size=4096)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    print(f"# 4096 sets: {n} keys", file=sys.stderr, flush=True)
+    d = splitmix64_keys_device(3, n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12, ctx=ctx)
+    ms = (time.perf_counter() - t0) * 1e3
+    print(f"# 4096 sets: {ms:.1f} ms", file=sys.stderr, flush=True)
+    ok = True
+    for i in range(0, 4096, 128):
+        k = d[int(off[i]):int(off[i + 1])].cpu().numpy().view(np.uint64)
+        ok &= bool(np.array_equal(np.sort(rss[i].query(k)), np.arange(sizes[i], dtype=np.uint64)))
+    return {"sets": 4096, "keys": n, "leaf": 12, "bucket": 100, "ms": ms, "value": ms * 1e6 / n,
+            "unit": "ns/object", "search_ms": st["search_ms"], "bijection_sample_ok": ok}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,8 +172,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", action="store_true", help="also run the VALU issue-rate and copy probes")
     ap.add_argument("--batch-sets", type=int, default=4, help="sets of n keys built together (0: skip)")
-    ap.add_argument("--serve-jobs", type=int, default=0,
-                    help="jobs of the served stream (0: skip; opt-in while the l=24 stream is under investigation)")
+    ap.add_argument("--serve-jobs", type=int, default=120, help="jobs of the served stream (0: skip)")
+    ap.add_argument("--sets4096", action="store_true", help="also run the 4096-set l=12 b=100 batched workload")
     ap.add_argument("--serve-time-limit", type=float, default=60.0, help="per-batch build time limit of the served stream (s)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -165,6 +183,7 @@ def main() -> int:
 
     import torch
     import gpurecsplit_amd as gra
+    from gpurecsplit_amd.keys import splitmix64_keys
 
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
@@ -237,6 +256,8 @@ def main() -> int:
     }
     if args.batch_sets > 1:
         line["batched"] = batched(gra, ctx, args, ms_per_step)
+    if args.sets4096:
+        line["batched_4096"] = batched_4096(gra, ctx)
     if args.serve_jobs > 0:
         line["served"] = served(args)
     if args.probe:

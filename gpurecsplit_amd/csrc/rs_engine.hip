@@ -184,11 +184,12 @@ using namespace rsa;
 
 // Launch configuration of the seed search, per node kind (index = NodeKind).
 // Defaults are the measured best on MI355X; RS_TUNE overrides them, e.g.
-// RS_TUNE="u=2,lut=1,g1=1,g2=2,g3=4,bpc=8" (u: candidates in flight per lane,
-// lut: LDS increment table for aggregation, gK: 64*u-candidate groups per claimed
-// chunk of kind K, bpc: workgroups of 4 waves per CU).
+// RS_TUNE="u=4,u0=1,lut=1,g1=1,g2=2,g3=4,bpc=8" (u: candidates in flight per lane
+// for every kind, uK: for kind K only, lut: LDS increment table for aggregation,
+// gK: 64*u-candidate groups per claimed chunk of kind K, bpc: workgroups of 4
+// waves per CU).  Fanout-2 nodes need ~40 candidates, so they run one per lane.
 struct Tune {
-    int u = 2;
+    int u[kNumKinds] = {1, 4, 4, 4, 4};
     int lut = 1;
     int groups[kNumKinds] = {1, 1, 2, 4, 4};
     int blocks_per_cu = 8;
@@ -207,7 +208,11 @@ static Tune parse_tune(const char *env) {
         if (e != std::string::npos) {
             const std::string k = kv.substr(0, e);
             const int v = std::atoi(kv.c_str() + e + 1);
-            if (k == "u" && (v == 1 || v == 2 || v == 4)) t.u = v;
+            const bool uv = v == 1 || v == 2 || v == 4;
+            if (k == "u" && uv)
+                for (int &x : t.u) x = v;
+            else if (k.size() == 2 && k[0] == 'u' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && uv)
+                t.u[k[1] - '0'] = v;
             else if (k == "lut") t.lut = v != 0;
             else if (k == "bpc" && v >= 1 && v <= 32) t.blocks_per_cu = v;
             else if (k.size() == 2 && k[0] == 'g' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && v >= 1 && v <= 64)
@@ -338,7 +343,7 @@ static void launch_u(rs_ctx *c, uint32_t kind, const uint64_t *keys, const Searc
 
 static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                           uint32_t blocks, size_t lds) {
-    switch (c->tune.u) {
+    switch (c->tune.u[kind]) {
     case 1: launch_u<1>(c, kind, keys, a, wide, lut, blocks, lds); break;
     case 4: launch_u<4>(c, kind, keys, a, wide, lut, blocks, lds); break;
     default: launch_u<2>(c, kind, keys, a, wide, lut, blocks, lds); break;
@@ -388,7 +393,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
 
     // ---- 0. time limit reference point
     unsigned long long *clock0 = nullptr, limit_ticks = 0;
-    if (c->time_limit_s > 0 && c->wall_khz > 0) {
+    if (c->time_limit_s > 0) {
         clock0 = c->clock.get<unsigned long long>(1);
         limit_ticks = std::max<unsigned long long>(1, (unsigned long long)(c->time_limit_s * c->wall_khz * 1e3));
         k_clock_mark<<<1, 64, 0, s>>>(clock0);
@@ -752,6 +757,7 @@ int rs_ctx_create(int device, rs_ctx **out) {
         c->num_cus = prop.multiProcessorCount;
         c->tune = parse_tune(getenv("RS_TUNE"));
         HIP_CHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
+        if (c->wall_khz <= 0) c->wall_khz = 100000;  // s_memrealtime: 100 MHz on gfx9 parts
         if (const char *tl = getenv("RS_TIME_LIMIT")) c->time_limit_s = std::max(0.0, atof(tl));
         HIP_CHECK(hipStreamCreate(&c->stream));  // blocking: ordered after null-stream work
         HIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * 1024, hipHostMallocDefault));

@@ -33,10 +33,14 @@ constexpr unsigned long long kNone = ~0ull;
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Wave-uniform values, kept in scalar registers.
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// __builtin_amdgcn_readfirstlane returns int: every result goes through uint32_t
+// before it is widened, or a low word with bit 31 set would be sign-extended into
+// the high word of a 64-bit value.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uni64(uint64_t v) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
@@ -52,6 +56,39 @@ __device__ __forceinline__ uint64_t bcast64_xor(uint64_t v, int mask) {
 __device__ __forceinline__ uint64_t bcast64_up(uint64_t v, int delta) {
     const uint32_t lo = __shfl_up((uint32_t)v, delta), hi = __shfl_up((uint32_t)(v >> 32), delta);
     return ((uint64_t)hi << 32) | lo;
+}
+
+// Search-side hash: fold_h16 (rs_common.h) split so that its last step fuses
+// with the consumer.  hash_pre is MurmurHash3 fmix32 of the folded word up to the
+// second multiply; h16_of finishes it, (t ^ (t >> 16)) & 0xFFFF, as one
+// word-select xor whose upper half is written as zero.
+__device__ __forceinline__ uint32_t hash_pre(uint64_t z) {
+    uint32_t t = (uint32_t)(z >> 32) ^ (uint32_t)z;
+    t ^= t >> 16;
+    t *= 0x85ebca6bu;
+    t ^= t >> 13;
+    return t * 0xc2b2ae35u;
+}
+__device__ __forceinline__ uint32_t h16_of(uint32_t t) {
+    uint32_t x;
+    __asm__("v_xor_b32_sdwa %0, %1, %1 dst_sel:WORD_0 dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:WORD_0"
+            : "=v"(x)
+            : "v"(t));
+    return x;
+}
+__device__ __forceinline__ uint32_t search_h16(uint64_t z) { return h16_of(hash_pre(z)); }
+
+// LDS byte address of entry (h16 * m) >> 16 of a word table that starts at word
+// `base_hi >> 16`: one 24-bit multiply-add, then the upper half of the product
+// shifted left by 2 in one word-select shift.  h16 * m < 2^28 and the table base
+// is below 2^12 words, so the sum fits 32 bits and its upper half is the index.
+__device__ __forceinline__ uint32_t lut_byte_addr(uint32_t h16, uint32_t m, uint32_t base_hi, uint32_t two) {
+    const uint32_t p = __umul24(h16, m) + base_hi;
+    uint32_t a;
+    __asm__("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+            : "=v"(a)
+            : "v"(two), "v"(p));
+    return a;
 }
 
 // Elias-Fano geometry (DoubleEF.hpp:63-69): q = 256 elements per sample,
@@ -314,7 +351,7 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
     }
     stream_keys(kp, 0, m, [&](uint64_t k) {
 #pragma unroll
-        for (int u = 0; u < U; u++) cnt[u] += fold_h16(k + s[u]) < H;
+        for (int u = 0; u < U; u++) cnt[u] += search_h16(k + s[u]) < H;
     });
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -333,7 +370,7 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
 template <int U, bool WIDE, bool LUT>
 __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask,
-                                               const uint32_t *lut, uint64_t u0) {
+                                               uint32_t lut_hi, uint32_t two, uint64_t u0) {
     using C = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
     const uint32_t lane = lane_id();
     uint64_t s[U];
@@ -346,11 +383,11 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
     stream_keys(kp, 0, m, [&](uint64_t k) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const uint32_t r = __umul24(fold_h16(k + s[u]), m) >> 16;
+            const uint32_t h = search_h16(k + s[u]);
             if (LUT) {
-                cnt[u] += lut[r];
+                cnt[u] += *(const __attribute__((address_space(3))) uint32_t *)(uintptr_t)lut_byte_addr(h, m, lut_hi, two);
             } else {
-                const uint32_t part = __umul24(r, magic) >> 20;
+                const uint32_t part = __umul24(__umul24(h, m) >> 16, magic) >> 20;
                 cnt[u] += (C)1 << __umul24(part, fb);
             }
         }
@@ -382,11 +419,11 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
     }
     stream_keys(kp, 0, nl, [&](uint64_t k) {
 #pragma unroll
-        for (int u = 0; u < U; u++) ml[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+        for (int u = 0; u < U; u++) ml[u] |= 1u << (__umul24(search_h16(k + sb[u]), L) >> 16);
     });
     stream_keys(kp, nl, L, [&](uint64_t k) {
 #pragma unroll
-        for (int u = 0; u < U; u++) mr[u] |= 1u << ((fold_h16(k + sb[u]) * L) >> 16);
+        for (int u = 0; u < U; u++) mr[u] |= 1u << (__umul24(search_h16(k + sb[u]), L) >> 16);
     });
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -430,7 +467,7 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
     }
     stream_keys(kp, 0, mid, [&](uint64_t k) {
 #pragma unroll
-        for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
+        for (int u = 0; u < U; u++) mk[u] |= 1u << (__umul24(search_h16(k + s[u]), m) >> 16);
     });
     if (mid < m) {
         bool any = false;
@@ -439,7 +476,7 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
         if (!__ballot(any)) return kNone;
         stream_keys(kp, mid, m, [&](uint64_t k) {
 #pragma unroll
-            for (int u = 0; u < U; u++) mk[u] |= 1u << ((fold_h16(k + s[u]) * m) >> 16);
+            for (int u = 0; u < U; u++) mk[u] |= 1u << (__umul24(search_h16(k + s[u]), m) >> 16);
         });
     }
 #pragma unroll
@@ -464,6 +501,10 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
     uint32_t *lut = s_lut + (threadIdx.x >> 6) * a.lut_stride;
+    // table start in LDS words, in the upper half (the low 32 bits of a flat LDS
+    // address are its LDS offset)
+    const uint32_t lut_hi = uni((((uint32_t)(uintptr_t)s_lut >> 2) + (threadIdx.x >> 6) * a.lut_stride) << 16);
+    const uint32_t two = 2;
     int node = -1;
     bool helping = false;
     uint32_t scan = wave;
@@ -594,7 +635,7 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             work += (uint64_t)U * 64 * m;
             if (KIND == kHigher) hit = group_higher<U>(kp, m, sconst, aux0, aux1, u0);
             else if (KIND == kUpper || KIND == kLower)
-                hit = group_aggr<U, WIDE, LUT>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut, u0);
+                hit = group_aggr<U, WIDE, LUT>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two, u0);
             else if (KIND == kLeafRot) hit = group_leaf_rot<U>(kp, m, aux0, aux1, u0);
             else hit = group_leaf_brute<U>(kp, m, aux0, u0);
             if (hit != kNone) break;

@@ -7,6 +7,9 @@
 //   kind 1  v_mul_lo_u32     (32-bit multiply, used twice per hash)
 //   kind 2  v_lshl_add_u64   (64-bit add of key and seed)
 //   kind 3  v_mul_u32_u24    (24-bit multiply)
+//   kind 4  v_xor_b32_sdwa   (word-select xor: the 16-bit fold of the hash)
+//   kind 5  v_mad_u32_u24    (24-bit multiply-add: remap plus table offset)
+//   kind 6  v_lshlrev_b32_sdwa (shift of a selected word: table address)
 // rs_probe_valu returns lane-ops per second; bench.py reports the ratio to kind 0
 // as the issue cost of each instruction in full-rate slots.
 #include <hip/hip_runtime.h>
@@ -36,6 +39,13 @@ __global__ __launch_bounds__(256) void k_probe(uint32_t iters, uint32_t b, uint3
                 if (KIND == 1) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a[j]) : "s"(b));
                 if (KIND == 2) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(x[j]) : "v"(y));
                 if (KIND == 3) asm volatile("v_mul_u32_u24 %0, %1, %0" : "+v"(a[j]) : "s"(b));
+                if (KIND == 4)
+                    asm volatile("v_xor_b32_sdwa %0, %0, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:DWORD"
+                                 : "+v"(a[j]) : "v"(a[(j + 1) & 7]));
+                if (KIND == 5) asm volatile("v_mad_u32_u24 %0, %0, %1, %2" : "+v"(a[j]) : "s"(b), "v"(a[(j + 1) & 7]));
+                if (KIND == 6)
+                    asm volatile("v_lshlrev_b32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1"
+                                 : "+v"(a[j]) : "v"(a[(j + 1) & 7]));
             }
         }
     }
@@ -52,7 +62,36 @@ __global__ __launch_bounds__(256) void k_probe_copy(const uint4 *__restrict__ sr
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Clock check: wall clock (s_memrealtime) and shader clock (s_memtime) before and
+// after a fixed amount of VALU work, from one lane of every workgroup.
+__global__ __launch_bounds__(64) void k_probe_clock(uint32_t iters, unsigned long long *out) {
+    const unsigned long long w0 = (unsigned long long)wall_clock64(), c0 = (unsigned long long)clock64();
+    uint32_t a = threadIdx.x;
+    for (uint32_t i = 0; i < iters; i++) asm volatile("v_mul_lo_u32 %0, %0, %0" : "+v"(a));
+    const unsigned long long w1 = (unsigned long long)wall_clock64(), c1 = (unsigned long long)clock64();
+    if (threadIdx.x == 0) {
+        out[4 * blockIdx.x + 0] = w0;
+        out[4 * blockIdx.x + 1] = w1;
+        out[4 * blockIdx.x + 2] = c0;
+        out[4 * blockIdx.x + 3] = c1 + (a & 1);
+    }
+}
+
 }  // namespace
+
+// Runs k_probe_clock on `blocks` workgroups; out holds 4 values per workgroup
+// (wall start, wall end, shader start, shader end); *khz the runtime's wall-clock rate.
+extern "C" int rs_probe_clock(int device, uint32_t blocks, uint32_t iters, unsigned long long *out, int *khz) {
+    if (hipSetDevice(device) != hipSuccess) return 2;
+    if (hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) return 2;
+    unsigned long long *d = nullptr;
+    if (hipMalloc(&d, (size_t)blocks * 32) != hipSuccess) return 7;
+    k_probe_clock<<<blocks, 64>>>(iters, d);
+    const bool ok = hipDeviceSynchronize() == hipSuccess &&
+                    hipMemcpy(out, d, (size_t)blocks * 32, hipMemcpyDeviceToHost) == hipSuccess;
+    (void)hipFree(d);
+    return ok ? 0 : 2;
+}
 
 // Copies `bytes` (a multiple of 16) between two device buffers with k_probe_copy.
 extern "C" int rs_probe_copy(int device, uint64_t bytes, double *ms_out) {
@@ -98,7 +137,10 @@ extern "C" int rs_probe_valu(int device, int kind, uint32_t iters, double *lane_
         case 0: k_probe<0><<<blocks, threads>>>(iters, 0x2545F491u, out); break;
         case 1: k_probe<1><<<blocks, threads>>>(iters, 0x2545F491u, out); break;
         case 2: k_probe<2><<<blocks, threads>>>(iters, 0x2545F491u, out); break;
-        default: k_probe<3><<<blocks, threads>>>(iters, 0x0045F491u, out); break;
+        case 3: k_probe<3><<<blocks, threads>>>(iters, 0x0045F491u, out); break;
+        case 4: k_probe<4><<<blocks, threads>>>(iters, 0x2545F491u, out); break;
+        case 5: k_probe<5><<<blocks, threads>>>(iters, 0x0045F491u, out); break;
+        default: k_probe<6><<<blocks, threads>>>(iters, 0x2545F491u, out); break;
         }
         (void)hipEventRecord(e1, 0);
         (void)hipEventSynchronize(e1);

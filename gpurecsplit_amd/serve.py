@@ -18,6 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import Context, GPURecSplit, build_batch
+from .keys import splitmix64_keys
 
 # HBM bytes per key of one build: input keys 16 B, bucket-sorted copy 8 B, plus
 # node descriptors, seeds and search state (~1 node per 4-8 keys, 58 B each).
@@ -179,16 +180,6 @@ def poisson_stream(num_jobs: int, rate_per_s: float, seed: int, size_lo: int, si
             n = min(n, size_cap_of_leaf(int(ls[i])))
         jobs.append((float(t[i]), n, int(ls[i]), bucket_of_leaf(int(ls[i])), int(rng.integers(1, 2**62))))
     return jobs
-
-
-def splitmix64_keys(seed: int, n: int) -> np.ndarray:
-    gamma = np.uint64(0x9E3779B97F4A7C15)
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed) + gamma * np.arange(1, 2 * n + 1, dtype=np.uint64)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return z.reshape(n, 2)
 
 
 def run_stream(stream, workers: int = 2, hbm_budget_bytes: int = 64 << 30, time_limit_s: float | None = None,

@@ -1,0 +1,46 @@
+"""Batched build in the shape of the 4096-set workload (l=12, b=100, per-set
+sizes log-uniform, size seed 3, keys from one SplitMix64(3) stream generated on
+the GPU), with sizes scaled to [1e3, 2e4] so that the host checks stay short:
+every set is a bijection onto [0, n_i), and sampled sets are byte-identical to
+the oracle's build of that set alone."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleMPHF
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gpurecsplit_amd as gra
+    from gpurecsplit_amd.keys import splitmix64_keys_device
+    return torch, gra, splitmix64_keys_device
+
+
+def test_4096_uneven_sets_l12_b100(env):
+    torch, gra, gen = env
+    rng = np.random.default_rng(3)
+    sizes = np.exp(rng.uniform(np.log(1_000), np.log(20_000), size=4096)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    print(f"4096 sets, {n} keys", flush=True)
+    d = gen(3, n)
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12)
+    assert len(rss) == 4096
+    assert st["num_buckets"] == int(sum(-(-int(s) // 100) for s in sizes))
+    keys = d.cpu().numpy().view(np.uint64)
+    del d
+    for i in range(4096):
+        k = keys[off[i]:off[i + 1]]
+        q = rss[i].query(k)
+        assert np.array_equal(np.sort(q), np.arange(sizes[i], dtype=np.uint64)), f"set {i}"
+    for i in range(0, 4096, 128):
+        k = keys[off[i]:off[i + 1]]
+        assert rss[i].serialize() == OracleMPHF.build(k, 12, 100, threads=8).serialize(), f"set {i}"
+    sys.stdout.flush()

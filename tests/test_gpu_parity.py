@@ -165,13 +165,33 @@ def test_batch_many_uneven_sets(torch_cuda):
 def test_time_limit_stops_build_and_context_recovers(torch_cuda):
     """A 1 us limit ends every search wave at its first claim (RS_ERR_TIMEOUT = 10);
     with the limit lifted the same context builds the oracle's bytes."""
-    keys = splitmix_keys(15, 20_000)
+    keys = splitmix_keys(15, 200_000)  # l=16, b=2000: tens of ms of search
     ctx = gra.Context(0, time_limit_s=1e-6)
     with pytest.raises(gra.RecSplitError) as e:
-        gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx)
+        gra.GPURecSplit(keys, 2000, leaf_size=16, ctx=ctx)
     assert e.value.code == 10
     ctx.set_time_limit(0)
-    rs = gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx)
-    assert rs.serialize() == OracleMPHF.build(keys, 12, 100).serialize()
+    rs = gra.GPURecSplit(keys, 2000, leaf_size=16, ctx=ctx)
+    assert rs.serialize() == OracleMPHF.build(keys, 16, 2000, threads=8).serialize()
     ctx.set_time_limit(600)  # a generous limit changes nothing
-    assert gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx).serialize() == rs.serialize()
+    assert gra.GPURecSplit(keys, 2000, leaf_size=16, ctx=ctx).serialize() == rs.serialize()
+
+
+def test_l24_leaf_seeds_above_2_31(torch_cuda):
+    """24-key rotation leaves whose lowest seeds lie above 2^31 and 2^32 (found by
+    the oracle): the search must stop at them (wave-uniform 64-bit values keep
+    their high word) and return the same bytes."""
+    for s in (100, 101, 103):  # oracle seeds 4433910963, 610147800, 2643051985
+        keys = splitmix_keys(s, 24)
+        ref = OracleMPHF.build(keys, 24, 24, threads=1)
+        rs = gra.GPURecSplit(keys, 24, leaf_size=24)
+        assert rs.serialize() == ref.serialize(), f"seed {s}"
+    assert max(OracleMPHF.build(splitmix_keys(100, 24), 24, 24).seeds()) >= 2**32
+
+
+def test_wide_counters_use_high_word(torch_cuda):
+    """l=20, one 900-key bucket: the upper node has 6 parts (5 counted 8-bit fields,
+    40 bits) and the lower nodes 8 parts (7 counted 5-bit fields, 35 bits), so the
+    64-bit counters carry fields above bit 31."""
+    keys = splitmix_keys(20, 900)
+    _parity(keys, 20, 900, threads=1)
