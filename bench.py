@@ -29,18 +29,19 @@ VALU_PEAK_TOPS = 157.3 / 2
 # VALU instructions per key-seed evaluation in the inner loop of each search
 # kernel (default launch: 4 candidates per lane, 1 for fanout-2 nodes; LDS
 # increment table), counted from the gfx950 assembly of librecsplit_amd.so, and
-# the same weighted by the issue cost measured by rs_probe.hip (v_mul_lo_u32 1.72,
-# v_lshl_add_u64 1.78, v_mul_u32_u24 / v_mad_u32_u24 1.61 v_xor_b32 slots).
+# the same weighted by the issue cost measured by rs_probe.hip in v_xor_b32 slots
+# (v_mul_lo_u32 1.65, v_lshl_add_u64 1.72, v_mul_u32_u24 1.54, v_xor_b32_sdwa 1.56,
+# v_mad_u32_u24 1.56, v_lshlrev_b32_sdwa 1.49, other ops 1).
 INSTR_PER_EVAL = {"fanout2": 11.5, "upper_aggr": 10.88, "lower_aggr": 10.88, "leaf_rotation": 10.94,
                   "leaf_bruteforce": 10.88}
-SLOTS_PER_EVAL = {"fanout2": 13.91, "upper_aggr": 13.75, "lower_aggr": 13.75, "leaf_rotation": 13.86,
-                  "leaf_bruteforce": 13.80}
-XOR_RATE = 5.747e13  # v_xor_b32 lane-ops/s measured by rs_probe.hip
+SLOTS_PER_EVAL = {"fanout2": 14.82, "upper_aggr": 15.11, "lower_aggr": 15.11, "leaf_rotation": 15.20,
+                  "leaf_bruteforce": 15.13}
+XOR_RATE = 5.50e13  # v_xor_b32 lane-ops/s measured by rs_probe.hip (5.38e13-5.75e13 over runs)
 
 # HBM bytes per launch of each search kernel at this workload, from two rocprofv3
 # --pmc runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, the factors of the 256 MiB copy
 # probe); profiles/k_search_plain_l16_b2000_n5M.md.
-TRAFFIC_BYTES = {"lower_aggr": 16.60e9, "upper_aggr": 4.55e9, "leaf_rotation": 10.66e9}
+TRAFFIC_BYTES = {"lower_aggr": 8.095e9, "upper_aggr": 2.083e9, "leaf_rotation": 4.820e9}
 
 
 def cpu_baseline(leaf: int, bucket: int, seed: int) -> dict:
