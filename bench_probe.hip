@@ -1,4 +1,5 @@
-// rs_probe.hip -- VALU issue-rate probe for the roofline of the seed search.
+// bench_probe.hip -- VALU issue-rate and copy probes used by bench.py --probe
+// (bench tooling, not part of the library).
 //
 // Each kernel runs 8 independent chains of one instruction kind per lane, 16
 // repetitions per loop trip, on a full grid (8 workgroups of 4 waves per CU), so

@@ -28,7 +28,7 @@ LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "bucket with 3000 or more keys",
     4: "duplicate keys", 5: "seed limit exceeded", 6: "Elias-Fano offset overflow", 7: "out of memory",
-    8: "malformed serialized stream", 9: "no device copy", 10: "time limit exceeded",
+    8: "malformed serialized stream", 9: "no device copy", 10: "time limit exceeded", 11: "internal error",
 }
 
 KIND_NAMES = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforce"]
@@ -228,12 +228,14 @@ class GPURecSplit:
 
     def query_device(self, d_keys, d_out=None):
         """Batched GPU queries: d_keys is a contiguous (n, 2) 64-bit CUDA tensor; returns
-        (or fills) an (n,) int64 CUDA tensor of results."""
+        (or fills) an (n,) int64 CUDA tensor of results.  A structure without a copy
+        on that GPU (deserialized, or from a batch) is uploaded on first use."""
         import torch
         if d_out is None:
             d_out = torch.empty(d_keys.shape[0], dtype=torch.int64, device=d_keys.device)
         _sync_torch(d_keys)
-        _check(lib.rs_query_device(self._ctx.h, self.h, C.c_void_p(d_keys.data_ptr()), d_keys.shape[0],
+        ctx = self._ctx or Context.default(d_keys.device.index or 0)
+        _check(lib.rs_query_device(ctx.h, self.h, C.c_void_p(d_keys.data_ptr()), d_keys.shape[0],
                                    C.c_void_p(d_out.data_ptr())))
         return d_out
 
@@ -279,3 +281,4 @@ def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: b
     out = [GPURecSplit(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=C.c_void_p(h))
            for h in handles]
     return out, st.as_dict()
+

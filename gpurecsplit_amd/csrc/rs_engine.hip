@@ -48,6 +48,12 @@ struct BuildError : std::runtime_error {
         if (e_ != hipSuccess) throw HipError(e_, #x, __LINE__); \
     } while (0)
 
+// Worst-case device bytes of a build of n keys in nb buckets (keys excluded):
+// 8 B/key of bucket-sorted keys, <= n tree nodes of 58 B (descriptor, size, seed,
+// search state), 8 B/key of Golomb-Rice words, 64 B/bucket of counts, offsets and
+// scans, 25 B/bucket + 1 B/key of Elias-Fano arrays, 1 MiB of fixed-size buffers.
+static uint64_t working_set_bound(uint64_t n, uint64_t nb) { return 75 * n + 89 * nb + (uint64_t(1) << 20); }
+
 // Grow-only device allocation reused across builds of a context.
 struct DevBuf {
     void *p = nullptr;
@@ -71,10 +77,13 @@ struct DevBuf {
     }
 };
 
+// Device-resident copy of a structure for batched queries; word counts are the
+// readable words of each array.
 struct DeviceCopy {
     int device = 0;
     uint64_t *rice = nullptr, *lower = nullptr, *upper_c = nullptr, *upper_p = nullptr, *jump = nullptr;
     uint32_t *memo = nullptr;
+    uint64_t n_rice = 0, n_lower = 0, n_uc = 0, n_up = 0, n_jump = 0;
 };
 
 void device_copy_free(DeviceCopy *d) {
@@ -184,12 +193,14 @@ using namespace rsa;
 
 // Launch configuration of the seed search, per node kind (index = NodeKind).
 // Defaults are the measured best on MI355X; RS_TUNE overrides them, e.g.
-// RS_TUNE="u=4,u0=1,lut=1,g1=1,g2=2,g3=4,bpc=8" (u: candidates in flight per lane
-// for every kind, uK: for kind K only, lut: LDS increment table for aggregation,
+// RS_TUNE="u=4,u0=1,lut=1,g1=1,g2=2,g3=4,bpc=8,sk=1" (u: candidates in flight per
+// lane for every kind, uK: for kind K only, lut: LDS increment table for aggregation,
+// sk / skK: keys in scalar registers for every kind / kind K,
 // gK: 64*u-candidate groups per claimed chunk of kind K, bpc: workgroups of 4
 // waves per CU).  Fanout-2 nodes need ~40 candidates, so they run one per lane.
 struct Tune {
     int u[kNumKinds] = {1, 4, 4, 4, 4};
+    int sk[kNumKinds] = {0, 0, 0, 0, 0};  // keys in scalar registers
     int lut = 1;
     int groups[kNumKinds] = {1, 1, 2, 4, 4};
     int blocks_per_cu = 8;
@@ -214,6 +225,10 @@ static Tune parse_tune(const char *env) {
             else if (k.size() == 2 && k[0] == 'u' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && uv)
                 t.u[k[1] - '0'] = v;
             else if (k == "lut") t.lut = v != 0;
+            else if (k == "sk")
+                for (int &x : t.sk) x = v != 0;
+            else if (k.size() == 3 && k[0] == 's' && k[1] == 'k' && k[2] >= '0' && k[2] < '0' + (int)kNumKinds)
+                t.sk[k[2] - '0'] = v != 0;
             else if (k == "bpc" && v >= 1 && v <= 32) t.blocks_per_cu = v;
             else if (k.size() == 2 && k[0] == 'g' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && v >= 1 && v <= 64)
                 t.groups[k[1] - '0'] = v;
@@ -242,46 +257,49 @@ struct rs_ctx {
 
 namespace rsa {
 
-static void scan_u64(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out) {
-    // out[0..n] = exclusive scan of in[0..n), out[n] = total
+// out[0..n] = exclusive scan of in[0..n), out[n] = total.  Block sums are
+// scanned recursively on the device; tmp holds every level's sums and offsets
+// (scan_tmp_words(n) words).
+static uint64_t scan_tmp_words(uint64_t n) {
+    uint64_t words = 0;
+    for (uint64_t items = n + 1;;) {
+        const uint64_t nblk = (items + kScanItems - 1) / kScanItems;
+        if (nblk <= 1) break;
+        words += 2 * nblk + 1;
+        items = nblk + 1;
+    }
+    return words + 1;
+}
+
+static void scan_level(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out, uint64_t *tmp) {
     const uint64_t items = n + 1;
     const uint64_t nblk = (items + kScanItems - 1) / kScanItems;
-    // temp: block sums + their scan (recursive levels are small)
-    uint64_t *tmp = c->scan_tmp.get<uint64_t>(4 * (nblk + 2) + 64);
-    uint64_t *sums = tmp, *offs = tmp + nblk + 1;
+    uint64_t *sums = tmp, *offs = tmp + nblk;  // nblk block sums, then their scan (nblk + 1)
     if (in32)
         k_scan_blocks<uint32_t><<<nblk, 256, 0, c->stream>>>((const uint32_t *)in, n, out, sums);
     else
         k_scan_blocks<uint64_t><<<nblk, 256, 0, c->stream>>>((const uint64_t *)in, n, out, sums);
     HIP_CHECK(hipGetLastError());
     if (nblk > 1) {
-        if (nblk + 1 > kScanItems) {
-            // second level larger than one workgroup: scan it on the host (rare: > 4M items)
-            std::vector<uint64_t> h(nblk);
-            HIP_CHECK(hipMemcpyAsync(h.data(), sums, nblk * 8, hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipStreamSynchronize(c->stream));
-            uint64_t run = 0;
-            for (auto &x : h) {
-                const uint64_t t = x;
-                x = run;
-                run += t;
-            }
-            HIP_CHECK(hipMemcpyAsync(offs, h.data(), nblk * 8, hipMemcpyHostToDevice, c->stream));
-            HIP_CHECK(hipStreamSynchronize(c->stream));
-        } else {
-            k_scan_blocks<uint64_t><<<1, 256, 0, c->stream>>>(sums, nblk, offs, offs + nblk + 1);
-            HIP_CHECK(hipGetLastError());
-        }
+        scan_level(c, sums, false, nblk, offs, tmp + 2 * nblk + 1);
         k_scan_add<<<(items + 255) / 256, 256, 0, c->stream>>>(out, n, offs);
         HIP_CHECK(hipGetLastError());
     }
 }
 
+static void scan_u64(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out) {
+    scan_level(c, in, in32, n, out, c->scan_tmp.get<uint64_t>(scan_tmp_words(n)));
+}
+
+// Device tables of a leaf size: the Golomb-Rice memo, then the unmasked node
+// count of a bucket by size (kMaxBucket entries each).
 static const uint32_t *ctx_memo(rs_ctx *c, uint32_t leaf) {
     if (!c->memo[leaf]) {
         const Memo &m = golomb_memo(leaf);
-        HIP_CHECK(hipMalloc(&c->memo[leaf], sizeof(uint32_t) * kMaxBucket));
+        const Memo &t = tree_nodes(leaf);
+        HIP_CHECK(hipMalloc(&c->memo[leaf], sizeof(uint32_t) * kMaxBucket * 2));
         HIP_CHECK(hipMemcpy(c->memo[leaf], m.data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->memo[leaf] + kMaxBucket, t.data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice));
     }
     return c->memo[leaf];
 }
@@ -316,38 +334,45 @@ static double ev_ms(rs_ctx *c, int a, int b) {
     return ms;
 }
 
-template <int KIND, int U>
+template <int KIND, int U, bool SK>
 static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
                         size_t lds) {
     hipStream_t s = c->stream;
     if (KIND == kUpper || KIND == kLower) {
-        if (wide) k_search<KIND, U, true, false><<<blocks, 256, 0, s>>>(keys, a);
-        else if (lut) k_search<KIND, U, false, true><<<blocks, 256, lds, s>>>(keys, a);
-        else k_search<KIND, U, false, false><<<blocks, 256, 0, s>>>(keys, a);
+        if (wide) k_search<KIND, U, true, false, SK><<<blocks, 256, 0, s>>>(keys, a);
+        else if (lut) k_search<KIND, U, false, true, SK><<<blocks, 256, lds, s>>>(keys, a);
+        else k_search<KIND, U, false, false, SK><<<blocks, 256, 0, s>>>(keys, a);
     } else {
-        k_search<KIND, U, false, false><<<blocks, 256, 0, s>>>(keys, a);
+        k_search<KIND, U, false, false, SK><<<blocks, 256, 0, s>>>(keys, a);
     }
 }
 
-template <int U>
+template <int U, bool SK>
 static void launch_u(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                      uint32_t blocks, size_t lds) {
     switch (kind) {
-    case kHigher: launch_kind<kHigher, U>(c, keys, a, wide, lut, blocks, lds); break;
-    case kUpper: launch_kind<kUpper, U>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLower: launch_kind<kLower, U>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLeafRot: launch_kind<kLeafRot, U>(c, keys, a, wide, lut, blocks, lds); break;
-    default: launch_kind<kLeafBrute, U>(c, keys, a, wide, lut, blocks, lds); break;
+    case kHigher: launch_kind<kHigher, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    case kUpper: launch_kind<kUpper, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLower: launch_kind<kLower, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLeafRot: launch_kind<kLeafRot, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    default: launch_kind<kLeafBrute, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    }
+}
+
+template <bool SK>
+static void launch_sk(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
+                      uint32_t blocks, size_t lds) {
+    switch (c->tune.u[kind]) {
+    case 1: launch_u<1, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    case 4: launch_u<4, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    default: launch_u<2, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
     }
 }
 
 static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                           uint32_t blocks, size_t lds) {
-    switch (c->tune.u[kind]) {
-    case 1: launch_u<1>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    case 4: launch_u<4>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    default: launch_u<2>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    }
+    if (c->tune.sk[kind]) launch_sk<true>(c, kind, keys, a, wide, lut, blocks, lds);
+    else launch_sk<false>(c, kind, keys, a, wide, lut, blocks, lds);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -359,13 +384,16 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
 
 constexpr uint64_t kTileKeys = 1 << 14;  // keys per histogram / scatter workgroup
 
-// Builds one MPHF per key set; set s owns keys [set_off[s], set_off[s+1]) of d_keys.
+// Builds one MPHF per key set; set s has set_n[s] keys at set_keys[s] (device).
 // All sets share one leaf size and bucket size; their buckets, tree nodes and
 // seed searches run together.  With device_copy, the (single) result also keeps
 // a device-resident copy for batched GPU queries.
-static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, const std::vector<uint64_t> &set_off,
-                                         uint32_t leaf, uint64_t bucket_size, bool rotate, bool device_copy,
+static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint64_t *> &set_keys,
+                                         const std::vector<uint64_t> &set_n, uint32_t leaf, uint64_t bucket_size,
+                                         bool rotate, bool device_copy,
                                          const std::chrono::steady_clock::time_point t_start) {
+    std::vector<uint64_t> set_off(set_n.size() + 1, 0);  // key offsets of the sets in the sorted array
+    for (size_t i = 0; i < set_n.size(); i++) set_off[i + 1] = set_off[i] + set_n[i];
     hipStream_t s = c->stream;
     c->n_ev = 0;
     const size_t S = set_off.size() - 1;
@@ -384,8 +412,8 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
         bbase[i] = NB;
         NB += nbs[i];
         max_nb = std::max(max_nb, nbs[i]);
-        for (uint64_t k = set_off[i]; k < set_off[i + 1]; k += kTileKeys)
-            tiles.push_back(KeyTile{k, std::min(k + kTileKeys, set_off[i + 1]), nbs[i], bbase[i]});
+        for (uint64_t k = 0; k < n; k += kTileKeys)
+            tiles.push_back(KeyTile{set_keys[i], k, std::min(k + kTileKeys, n), nbs[i], bbase[i]});
     }
     bbase[S] = NB;
     const uint64_t n_total = set_off[S] - set_off[0];
@@ -415,11 +443,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     HIP_CHECK(hipMemsetAsync(size_hist, 0, (kMaxBucket + 1) * 4, s));
     HIP_CHECK(hipMemsetAsync(err, 0, 16, s));
     if (max_nb <= 8192)
-        k_bucket_hist<true><<<tiles.size(), 256, max_nb * 4, s>>>(d_keys, d_tiles, counts);
+        k_bucket_hist<true><<<tiles.size(), 256, max_nb * 4, s>>>(d_tiles, counts);
     else
-        k_bucket_hist<false><<<tiles.size(), 256, 0, s>>>(d_keys, d_tiles, counts);
+        k_bucket_hist<false><<<tiles.size(), 256, 0, s>>>(d_tiles, counts);
     HIP_CHECK(hipGetLastError());
-    k_bucket_meta<<<(NB + 255) / 256, 256, 0, s>>>(counts, NB, memo, counts64, node_cnt, size_hist);
+    k_bucket_meta<<<(NB + 255) / 256, 256, 0, s>>>(counts, NB, memo + kMaxBucket, counts64, node_cnt, size_hist);
     HIP_CHECK(hipGetLastError());
     scan_u64(c, counts, true, NB, acc);
     scan_u64(c, node_cnt, false, NB, node_base);
@@ -451,7 +479,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     uint64_t *sorted = c->sorted.get<uint64_t>(n_total + 8);  // +8: look-ahead loads of the search
     uint32_t *cursor = c->cursor.get<uint32_t>(NB);
     HIP_CHECK(hipMemsetAsync(cursor, 0, NB * 4, s));
-    k_scatter<<<tiles.size(), 256, 0, s>>>(d_keys, d_tiles, acc, cursor, sorted);
+    k_scatter<<<tiles.size(), 256, 0, s>>>(d_tiles, acc, cursor, sorted);
     HIP_CHECK(hipGetLastError());
     k_sort_buckets<<<NB, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err, rotate ? leaf : 0);
     HIP_CHECK(hipGetLastError());
@@ -469,8 +497,12 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     k_expand<<<(NB + 63) / 64, 64, 0, s>>>(acc, node_base, NB, ea);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync((uint64_t *)h + 1, node_base + NB, 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[0] & 1) throw BuildError(RS_ERR_DUPLICATE_KEYS, "two keys of one bucket share the same `second` half");
+    if (((uint64_t *)h)[1] != total_nodes)
+        throw BuildError(RS_ERR_INTERNAL, "seed slots " + std::to_string(((uint64_t *)h)[1]) + " != tree nodes " +
+                                              std::to_string(total_nodes));
     const int e_part1 = ev_record(c);
 
     // ---- 4. search phases
@@ -545,7 +577,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
     uint64_t *pos = c->pos.get<uint64_t>(NB + 1);
     unsigned long long *evals = c->evals.get<unsigned long long>(kNumKinds);
     HIP_CHECK(hipMemsetAsync(evals, 0, kNumKinds * 8, s));
-    EncodeArgs ca{acc, node_base, seeds, node_m, memo, NB, P, rotate ? 1 : 0};
+    EncodeArgs ca{acc, node_base, seeds, node_m, memo, memo + kMaxBucket, NB, P, rotate ? 1 : 0};
     k_bucket_bits<<<(NB + 3) / 4, 256, 0, s>>>(ca, bits, evals);
     HIP_CHECK(hipGetLastError());
     scan_u64(c, bits, false, NB, pos);
@@ -596,6 +628,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
 
     // ---- 7. Elias-Fano: per-set widths and sizes (DoubleEF.hpp:181-202)
     std::vector<EliasFano> efs(S);
+    std::vector<std::array<uint64_t, 4>> ef_words(S);  // lower, upper (keys), upper (positions), jump
     uint64_t tot_lower = 0, tot_uc = 0, tot_up = 0, tot_jump = 0;
     for (size_t i = 0; i < S; i++) {
         EliasFano &ef = efs[i];
@@ -615,14 +648,18 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
         enc[i].up_word = tot_up;
         enc[i].jump_word = tot_jump;
         enc[i].jump_words = ef_jump_words(nb);
-        ef.lower_bits.assign(ef_lower_words(nb, ef.l_cum_keys, ef.l_position), 0);
-        ef.upper_cum.assign(ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys), 0);
-        ef.upper_pos.assign(ef_upper_words(nb, ef.u_position, ef.l_position), 0);
-        ef.jump.assign(enc[i].jump_words, 0);
-        tot_lower += ef.lower_bits.size();
-        tot_uc += ef.upper_cum.size();
-        tot_up += ef.upper_pos.size();
-        tot_jump += ef.jump.size();
+        ef_words[i][0] = ef_lower_words(nb, ef.l_cum_keys, ef.l_position);
+        ef_words[i][1] = ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys);
+        ef_words[i][2] = ef_upper_words(nb, ef.u_position, ef.l_position);
+        ef_words[i][3] = enc[i].jump_words;
+        ef.lower_bits.assign(ef_words[i][0], 0);
+        ef.upper_cum.assign(ef_words[i][1], 0);
+        ef.upper_pos.assign(ef_words[i][2], 0);
+        ef.jump.assign(ef_words[i][3], 0);
+        tot_lower += ef_words[i][0];
+        tot_uc += ef_words[i][1];
+        tot_up += ef_words[i][2];
+        tot_jump += ef_words[i][3];
     }
     HIP_CHECK(hipMemcpyAsync(d_enc, enc.data(), S * sizeof(SetEnc), hipMemcpyHostToDevice, s));
     unsigned long long *d_lower = c->ef_lower.get<unsigned long long>(tot_lower + 1);
@@ -662,6 +699,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
         HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (ef.upper_pos.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, ef.jump.size() * 8 + 8, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync(dc->memo, memo, kMaxBucket * 4, hipMemcpyDeviceToDevice, s));
+        dc->n_rice = rice_total;
+        dc->n_lower = ef.lower_bits.size();
+        dc->n_uc = ef.upper_cum.size();
+        dc->n_up = ef.upper_pos.size();
+        dc->n_jump = ef.jump.size();
     }
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[16] & 4) {
@@ -707,7 +749,16 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const uint64_t *d_keys, cons
 
 static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t leaf, uint64_t bucket_size,
                       bool rotate, const std::chrono::steady_clock::time_point t_start) {
-    return build_sets(c, d_keys, {0, n}, leaf, bucket_size, rotate, true, t_start)[0];
+    return build_sets(c, {d_keys}, {n}, leaf, bucket_size, rotate, true, t_start)[0];
+}
+
+// Every reusable device buffer of a context.
+static std::vector<DevBuf *> ctx_bufs(rs_ctx *c) {
+    return {&c->keys_in, &c->counts, &c->counts64, &c->cursor, &c->acc, &c->node_cnt, &c->node_base,
+            &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
+            &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
+            &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
+            &c->set_gather, &c->set_enc, &c->bit_off, &c->clock};
 }
 
 static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn) {
@@ -728,6 +779,48 @@ static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn)
     }
 }
 
+// The structure's device copy on ctx's GPU, made from its host arrays on first
+// use (deserialized handles and batch results have none).  A copy on another
+// device is replaced.  Uploads of one handle are serialised by a process-wide lock.
+static const DeviceCopy *device_copy_of(rs_ctx *c, rs_mphf *h) {
+    static std::mutex upload_mu;
+    std::lock_guard<std::mutex> g(upload_mu);
+    if (h->dev && h->dev->device == c->device) return h->dev;
+    if (h->dev) {
+        device_copy_free(h->dev);
+        h->dev = nullptr;
+    }
+    DeviceCopy *dc = new DeviceCopy();
+    dc->device = c->device;
+    const EliasFano &ef = h->ef;
+    auto up = [&](uint64_t **dst, const uint64_t *src, uint64_t words) {
+        // one zero word of padding beyond the readable words
+        HIP_CHECK(hipMalloc(dst, (words + 1) * 8));
+        HIP_CHECK(hipMemsetAsync(*dst, 0, (words + 1) * 8, c->stream));
+        if (words) HIP_CHECK(hipMemcpyAsync(*dst, src, words * 8, hipMemcpyHostToDevice, c->stream));
+    };
+    try {
+        up(&dc->rice, h->rice.data(), h->rice_words);
+        up(&dc->lower, ef.lower_bits.data(), ef.lower_bits.size());
+        up(&dc->upper_c, ef.upper_cum.data(), ef.upper_cum.size());
+        up(&dc->upper_p, ef.upper_pos.data(), ef.upper_pos.size());
+        up(&dc->jump, ef.jump.data(), ef.jump.size());
+        HIP_CHECK(hipMalloc(&dc->memo, kMaxBucket * 4));
+        HIP_CHECK(hipMemcpyAsync(dc->memo, h->memo->data(), kMaxBucket * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        device_copy_free(dc);
+        throw;
+    }
+    dc->n_rice = h->rice_words;
+    dc->n_lower = ef.lower_bits.size();
+    dc->n_uc = ef.upper_cum.size();
+    dc->n_up = ef.upper_pos.size();
+    dc->n_jump = ef.jump.size();
+    h->dev = dc;
+    return dc;
+}
+
 static int check_args(rs_ctx *c, const void *keys, size_t n, uint32_t leaf, size_t bucket, rs_mphf **out) {
     if (!c || !keys || !out) return set_error("null argument"), RS_ERR_ARG;
     if (n == 0) return set_error("no keys"), RS_ERR_ARG;
@@ -736,6 +829,17 @@ static int check_args(rs_ctx *c, const void *keys, size_t n, uint32_t leaf, size
     if (n >= (uint64_t(1) << 40)) return set_error("more than 2^40 keys"), RS_ERR_ARG;
     return RS_OK;
 }
+
+// Hooks of the construction server (rs_server.cpp).
+int build_sets_status(rs_ctx *c, const std::vector<const uint64_t *> &set_keys, const std::vector<uint64_t> &set_n,
+                      uint32_t leaf, uint64_t bucket, bool rotate, std::vector<rs_mphf *> &out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    return guarded("build", c, [&] { out = build_sets(c, set_keys, set_n, leaf, bucket, rotate, false, t0); });
+}
+
+uint64_t build_bound_bytes(uint64_t n, uint64_t nb) { return working_set_bound(n, nb); }
+
+int ctx_device(const rs_ctx *c) { return c->device; }
 
 }  // namespace rsa
 
@@ -781,16 +885,29 @@ int rs_ctx_set_time_limit(rs_ctx *c, double seconds) {
     return RS_OK;
 }
 
+uint64_t rs_ctx_device_bytes(rs_ctx *c) {
+    if (!c) return 0;
+    std::lock_guard<std::mutex> g(c->mu);
+    uint64_t t = 0;
+    for (DevBuf *b : ctx_bufs(c)) t += b->cap;
+    for (auto &m : c->memo)
+        if (m) t += sizeof(uint32_t) * kMaxBucket * 2;
+    return t;
+}
+
+int rs_ctx_release(rs_ctx *c) {
+    if (!c) return set_error("rs_ctx_release: null context"), RS_ERR_ARG;
+    return guarded("rs_ctx_release", c, [&] {
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        for (DevBuf *b : ctx_bufs(c)) b->release();
+    });
+}
+
 void rs_ctx_destroy(rs_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (DevBuf *b : {&c->keys_in, &c->counts, &c->counts64, &c->cursor, &c->acc, &c->node_cnt, &c->node_base,
-                      &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
-                      &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
-                      &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
-                      &c->set_gather, &c->set_enc, &c->bit_off, &c->clock})
-        b->release();
+    for (DevBuf *b : ctx_bufs(c)) b->release();
     for (auto &m : c->memo)
         if (m) (void)hipFree(m);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
@@ -824,8 +941,13 @@ int rs_build_batch(rs_ctx *c, const uint64_t *d_keys, const uint64_t *set_offset
         if (set_offsets[i + 1] <= set_offsets[i]) return set_error("rs_build_batch: every key set needs a key"), RS_ERR_ARG;
     if (int e = check_args(c, d_keys, set_offsets[num_sets] - set_offsets[0], leaf, bucket, out)) return e;
     return guarded("rs_build_batch", c, [&] {
-        std::vector<uint64_t> off(set_offsets, set_offsets + num_sets + 1);
-        std::vector<rs_mphf *> r = build_sets(c, d_keys, off, leaf, bucket, rotate != 0, false, t0);
+        std::vector<const uint64_t *> ptrs(num_sets);
+        std::vector<uint64_t> ns(num_sets);
+        for (size_t i = 0; i < num_sets; i++) {
+            ptrs[i] = d_keys + 2 * set_offsets[i];
+            ns[i] = set_offsets[i + 1] - set_offsets[i];
+        }
+        std::vector<rs_mphf *> r = build_sets(c, ptrs, ns, leaf, bucket, rotate != 0, false, t0);
         for (size_t i = 0; i < num_sets; i++) out[i] = r[i];
         if (stats) *stats = r[0]->stats;
     });
@@ -833,17 +955,22 @@ int rs_build_batch(rs_ctx *c, const uint64_t *d_keys, const uint64_t *set_offset
 
 int rs_query_device(rs_ctx *c, const rs_mphf *h, const uint64_t *d_keys, size_t n, uint64_t *d_out) {
     if (!c || !h || (!d_keys && n) || (!d_out && n)) return set_error("rs_query_device: null argument"), RS_ERR_ARG;
-    if (!h->dev || h->dev->device != c->device)
-        return set_error("rs_query_device: structure has no copy on this device"), RS_ERR_NO_DEVICE;
     if (n == 0) return RS_OK;
     return guarded("rs_query_device", c, [&] {
+        const DeviceCopy *dc = device_copy_of(c, const_cast<rs_mphf *>(h));
         QueryArgs q{};
-        q.rice = h->dev->rice;
-        q.lower = h->dev->lower;
-        q.upper_c = h->dev->upper_c;
-        q.upper_p = h->dev->upper_p;
-        q.jump = h->dev->jump;
-        q.memo = h->dev->memo;
+        q.rice = dc->rice;
+        q.lower = dc->lower;
+        q.upper_c = dc->upper_c;
+        q.upper_p = dc->upper_p;
+        q.jump = dc->jump;
+        q.memo = dc->memo;
+        q.n_rice = dc->n_rice;
+        q.n_lower = dc->n_lower;
+        q.n_uc = dc->n_uc;
+        q.n_up = dc->n_up;
+        q.n_jump = dc->n_jump;
+        q.keys_count = h->keys_count;
         q.nbuckets = h->nbuckets;
         q.bpk = h->ef.bits_per_key_fixed_point;
         q.lc = h->ef.l_cum_keys;

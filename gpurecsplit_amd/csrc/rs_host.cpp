@@ -72,6 +72,31 @@ const Memo &golomb_memo(uint32_t leaf) {
     return *tables[leaf];
 }
 
+static Memo build_tree_nodes(uint32_t leaf) {
+    const SplitParams P = split_params(leaf);
+    Memo nodes{};
+    for (uint32_t s = 2; s < kMaxBucket; ++s) {
+        if (s <= leaf) {
+            nodes[s] = 1;
+            continue;
+        }
+        uint32_t k[kMaxFanout];
+        const uint32_t fan = children(P, s, k);
+        uint32_t t = 1;
+        for (uint32_t i = 0; i < fan; ++i) t += nodes[k[i]];
+        nodes[s] = t;
+    }
+    return nodes;
+}
+
+const Memo &tree_nodes(uint32_t leaf) {
+    static std::mutex mu;
+    static Memo *tables[kMaxLeaf + 1] = {};
+    std::lock_guard<std::mutex> g(mu);
+    if (!tables[leaf]) tables[leaf] = new Memo(build_tree_nodes(leaf));
+    return *tables[leaf];
+}
+
 // ---------------------------------------------------------------- Elias-Fano
 
 void EliasFano::derive_widths() {
@@ -91,7 +116,10 @@ static inline uint64_t load_u64_at_byte(const std::vector<uint64_t> &v, uint64_t
     return t;
 }
 
-// DoubleEF.hpp:367-402
+// DoubleEF.hpp:367-402.  Word reads past the end of an array (possible only for
+// a malformed deserialized structure) read all-ones words, so every scan ends.
+static inline uint64_t word_at(const std::vector<uint64_t> &v, uint64_t i) { return i < v.size() ? v[i] : ~uint64_t(0); }
+
 void EliasFano::get(uint64_t i, uint64_t &cum_keys, uint64_t &cum_keys_next, uint64_t &position) const {
     const uint64_t lc = l_cum_keys, lp = l_position;
     const uint64_t mask_c = (uint64_t(1) << lc) - 1, mask_p = (uint64_t(1) << lp) - 1;
@@ -99,16 +127,15 @@ void EliasFano::get(uint64_t i, uint64_t &cum_keys, uint64_t &cum_keys_next, uin
     uint64_t lower = load_u64_at_byte(lower_bits, pos_lower / 8) >> (pos_lower % 8);
     const uint64_t jsq = (i / kEfSuperQ) * kEfSuperSize * 2;
     const uint64_t jin = (i % kEfSuperQ) / kEfQ;
-    uint16_t off[2];
-    memcpy(off, (const uint8_t *)(jump.data() + jsq + 2) + 4 * jin, 4);
-    const uint64_t jump_c = jump[jsq] + off[0];
-    const uint64_t jump_p = jump[jsq + 1] + off[1];
+    const uint64_t pair = load_u64_at_byte(jump, (jsq + 2) * 8 + 4 * jin);
+    const uint64_t jump_c = word_at(jump, jsq) + (pair & 0xFFFF);
+    const uint64_t jump_p = word_at(jump, jsq + 1) + ((pair >> 16) & 0xFFFF);
     uint64_t wc = jump_c / 64, wp = jump_p / 64;
-    uint64_t win_c = upper_cum[wc] & (~uint64_t(0) << (jump_c % 64));
-    uint64_t win_p = upper_pos[wp] & (~uint64_t(0) << (jump_p % 64));
+    uint64_t win_c = word_at(upper_cum, wc) & (~uint64_t(0) << (jump_c % 64));
+    uint64_t win_p = word_at(upper_pos, wp) & (~uint64_t(0) << (jump_p % 64));
     uint64_t dc = i & kEfQMask, dp = i & kEfQMask;
-    for (uint64_t c; (c = (uint64_t)__builtin_popcountll(win_c)) <= dc; dc -= c) win_c = upper_cum[++wc];
-    for (uint64_t c; (c = (uint64_t)__builtin_popcountll(win_p)) <= dp; dp -= c) win_p = upper_pos[++wp];
+    for (uint64_t c; (c = (uint64_t)__builtin_popcountll(win_c)) <= dc; dc -= c) win_c = word_at(upper_cum, ++wc);
+    for (uint64_t c; (c = (uint64_t)__builtin_popcountll(win_p)) <= dp; dp -= c) win_p = word_at(upper_pos, ++wp);
     const uint64_t sel_c = select_in_word(win_c, dc);
     const uint64_t cum_delta = i * (uint64_t)cum_keys_min_delta;
     cum_keys = ((wc * 64 + sel_c - i) << lc | (lower & mask_c)) + cum_delta;
@@ -117,7 +144,7 @@ void EliasFano::get(uint64_t i, uint64_t &cum_keys, uint64_t &cum_keys_next, uin
     position = ((wp * 64 + select_in_word(win_p, dp) - i) << lp | (lower & mask_p)) + bit_delta +
                (uint64_t)(int64_t)(bits_per_key_fixed_point * cum_keys >> 20);
     win_c &= (~uint64_t(0) << sel_c) << 1;
-    while (win_c == 0) win_c = upper_cum[++wc];
+    while (win_c == 0) win_c = word_at(upper_cum, ++wc);
     lower >>= lp;
     cum_keys_next = ((wc * 64 + (uint64_t)__builtin_ctzll(win_c) - i - 1) << lc | (lower & mask_c)) + cum_delta +
                     (uint64_t)cum_keys_min_delta;
@@ -135,20 +162,25 @@ uint64_t EliasFano::bit_count_position() const {
 
 // Golomb-Rice descriptor reader (RiceBitVector.hpp:155-218): fixed parts are
 // read at curr_fixed, unary parts from a 64-bit window advancing over the words.
+// Reads past the last word (a malformed deserialized structure only) see
+// all-ones words, so every scan ends inside the buffer.
 struct RiceReader {
     const uint64_t *data;
+    uint64_t words;  // readable words
     uint64_t fixed_off = 0;
     uint64_t window = 0;
-    const uint64_t *uptr = nullptr;
+    uint64_t uidx = 0;  // next unary word
     int valid = 0;
 
-    explicit RiceReader(const uint64_t *d) : data(d) {}
+    RiceReader(const uint64_t *d, uint64_t n) : data(d), words(n) {}
+
+    uint64_t word(uint64_t i) const { return i < words ? data[i] : ~uint64_t(0); }
 
     void reset(uint64_t bit_pos, uint64_t unary_offset) {
         fixed_off = bit_pos;
         const uint64_t up = bit_pos + unary_offset;
-        uptr = data + up / 64;
-        window = *(uptr++) >> (up & 63);
+        uidx = up / 64;
+        window = word(uidx++) >> (up & 63);
         valid = 64 - (int)(up & 63);
     }
 
@@ -156,11 +188,11 @@ struct RiceReader {
         uint64_t result = 0;
         if (window == 0) {
             result += (uint64_t)valid;
-            window = *(uptr++);
+            window = word(uidx++);
             valid = 64;
             while (window == 0) {
                 result += 64;
-                window = *(uptr++);
+                window = word(uidx++);
             }
         }
         const uint64_t pos = (uint64_t)__builtin_ctzll(window);
@@ -169,9 +201,9 @@ struct RiceReader {
         valid -= (int)pos + 1;
         result += pos;
         result <<= log2golomb;
-        uint64_t fixed;
-        memcpy(&fixed, (const uint8_t *)data + fixed_off / 8, 8);
-        result |= (fixed >> (fixed_off % 8)) & ((uint64_t(1) << log2golomb) - 1);
+        const uint64_t w = fixed_off / 64, sh = fixed_off % 64;
+        const uint64_t fixed = sh ? (word(w) >> sh) | (word(w + 1) << (64 - sh)) : word(w);
+        result |= fixed & ((uint64_t(1) << log2golomb) - 1);
         fixed_off += (uint64_t)log2golomb;
         return result;
     }
@@ -179,7 +211,7 @@ struct RiceReader {
     void skip_subtree(uint64_t nodes, uint64_t fixed_len) {
         uint64_t missing = nodes, cnt;
         while ((cnt = (uint64_t)__builtin_popcountll(window)) < missing) {
-            window = *(uptr++);
+            window = word(uidx++);
             missing -= cnt;
             valid = 64;
         }
@@ -203,7 +235,8 @@ uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     uint64_t cum_keys, cum_keys_next, bit_pos;
     h.ef.get(bucket, cum_keys, cum_keys_next, bit_pos);
     uint64_t m = cum_keys_next - cum_keys;
-    RiceReader rd(h.rice.data());
+    if (m >= kMaxBucket) return h.keys_count;  // malformed structure: no valid answer
+    RiceReader rd(h.rice.data(), h.rice.size());
     rd.reset(bit_pos, skip_bits(memo, m));
     int level = 0;
     while (m > P.upper) {
@@ -378,13 +411,40 @@ int rs_mphf_deserialize(const uint8_t *buf, size_t len, uint32_t leaf_size, int 
     r.vec(h->ef.upper_cum, 0);
     r.vec(h->ef.upper_pos, 0);
     r.vec(h->ef.jump, 0);
-    if (!r.ok || h->bucket_size == 0) {
+    std::string bad;
+    if (!r.ok) bad = "truncated stream";
+    else if (h->bucket_size == 0 || h->keys_count == 0) bad = "zero bucket size or key count";
+    else if (h->rice_words == 0) bad = "empty descriptor vector";
+    if (bad.empty()) {
+        h->nbuckets = std::max<uint64_t>(1, (h->keys_count + h->bucket_size - 1) / h->bucket_size);
+        h->ef.derive_widths();
+        const EliasFano &ef = h->ef;
+        const uint64_t nb = h->nbuckets;
+        if (ef.num_buckets != nb)
+            bad = "Elias-Fano bucket count " + std::to_string(ef.num_buckets) + " != ceil(keys / bucket size) " +
+                  std::to_string(nb);
+        else if (ef.l_cum_keys + ef.l_position > 56)
+            bad = "Elias-Fano lower-bit widths out of range";
+        else if (ef.lower_bits.size() != ef_lower_words(nb, ef.l_cum_keys, ef.l_position) ||
+                 ef.upper_cum.size() != ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys) ||
+                 ef.upper_pos.size() != ef_upper_words(nb, ef.u_position, ef.l_position) ||
+                 ef.jump.size() != ef_jump_words(nb))
+            bad = "Elias-Fano array sizes do not match its parameters";
+    }
+    if (bad.empty()) {
+        // first and last bucket: offsets start at 0, keys add up, and the last
+        // bucket's descriptor starts inside the descriptor vector
+        uint64_t c0, c1, p0, cl, cn, pl;
+        h->ef.get(0, c0, c1, p0);
+        h->ef.get(h->nbuckets - 1, cl, cn, pl);
+        if (c0 != 0 || p0 != 0 || cn != h->keys_count || cl > cn || pl >= h->rice_words * 64)
+            bad = "Elias-Fano offsets inconsistent with the key count or descriptor length";
+    }
+    if (!bad.empty()) {
         delete h;
-        set_error("rs_mphf_deserialize: truncated or malformed stream");
+        set_error("rs_mphf_deserialize: malformed stream: " + bad);
         return RS_ERR_FORMAT;
     }
-    h->nbuckets = std::max<uint64_t>(1, (h->keys_count + h->bucket_size - 1) / h->bucket_size);
-    h->ef.derive_widths();
     h->memo = &golomb_memo(leaf_size);
     h->P = split_params(leaf_size);
     *out = h;

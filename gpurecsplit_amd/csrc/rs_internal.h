@@ -17,6 +17,12 @@ using Memo = std::array<uint32_t, kMaxBucket>;
 const Memo &golomb_memo(uint32_t leaf);
 SplitParams split_params(uint32_t leaf);
 
+// Nodes of the split tree of a bucket of m keys (0 for m < 2).  The memo's 11-bit
+// node field wraps for leaf size 2 and buckets of 2049 or more keys (the reference
+// skips its assert for leaf sizes below 3, AbstractParallelRecSplit.hpp:217); the
+// build sizes its per-bucket seed slots from this unmasked table instead.
+const Memo &tree_nodes(uint32_t leaf);
+
 inline uint32_t golomb_param(const Memo &memo, uint64_t m) { return memo[m] >> 27; }
 inline uint32_t skip_bits(const Memo &memo, uint64_t m) { return memo[m] & 0xFFFF; }
 inline uint32_t skip_nodes(const Memo &memo, uint64_t m) { return (memo[m] >> 16) & 0x7FF; }
@@ -68,4 +74,12 @@ struct rs_mphf {
 namespace rsa {
 void set_error(const std::string &msg);
 void device_copy_free(DeviceCopy *d);
+
+// Builds one MPHF per key set on ctx (set i: set_n[i] keys at set_keys[i], device
+// memory); returns an rs_status and, on success, the handles in out.
+int build_sets_status(rs_ctx *c, const std::vector<const uint64_t *> &set_keys, const std::vector<uint64_t> &set_n,
+                      uint32_t leaf, uint64_t bucket, bool rotate, std::vector<rs_mphf *> &out);
+// Upper bound of the device bytes a build of n keys in nb buckets uses.
+uint64_t build_bound_bytes(uint64_t n, uint64_t nb);
+int ctx_device(const rs_ctx *c);
 }  // namespace rsa

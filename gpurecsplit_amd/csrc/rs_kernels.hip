@@ -62,13 +62,13 @@ __device__ __forceinline__ uint64_t bcast64_up(uint64_t v, int delta) {
 // with the consumer.  hash_pre is MurmurHash3 fmix32 of the folded word up to the
 // second multiply; h16_of finishes it, (t ^ (t >> 16)) & 0xFFFF, as one
 // word-select xor whose upper half is written as zero.
-__device__ __forceinline__ uint32_t hash_pre(uint64_t z) {
-    uint32_t t = (uint32_t)(z >> 32) ^ (uint32_t)z;
+__device__ __forceinline__ uint32_t hash_pre32(uint32_t t) {
     t ^= t >> 16;
     t *= 0x85ebca6bu;
     t ^= t >> 13;
     return t * 0xc2b2ae35u;
 }
+__device__ __forceinline__ uint32_t hash_pre(uint64_t z) { return hash_pre32((uint32_t)(z >> 32) ^ (uint32_t)z); }
 __device__ __forceinline__ uint32_t h16_of(uint32_t t) {
     uint32_t x;
     __asm__("v_xor_b32_sdwa %0, %1, %1 dst_sel:WORD_0 dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:WORD_0"
@@ -104,19 +104,21 @@ __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long lon
 
 // A contiguous range of keys of one key set: its keys map to buckets
 // bbase + floor(first * nb / 2^64) (common.hpp:273-280, with the set's own nb).
+// Every set's keys may live in their own device allocation.
 struct KeyTile {
-    uint64_t k0, k1;  // key range [k0, k1)
-    uint64_t nb;      // buckets of the set
-    uint64_t bbase;   // first global bucket of the set
+    const uint64_t *keys;  // the set's (first, second) pairs
+    uint64_t k0, k1;       // key range [k0, k1) of the set
+    uint64_t nb;           // buckets of the set
+    uint64_t bbase;        // first global bucket of the set
 };
 
 // Bucket histogram, one workgroup per key tile.  Sets with few buckets count in
 // LDS first (PRIV: every tile's set has at most 8192 buckets).
 template <bool PRIV>
-__global__ __launch_bounds__(256) void k_bucket_hist(const uint64_t *__restrict__ keys, const KeyTile *__restrict__ tiles,
-                                                     uint32_t *__restrict__ counts) {
+__global__ __launch_bounds__(256) void k_bucket_hist(const KeyTile *__restrict__ tiles, uint32_t *__restrict__ counts) {
     extern __shared__ uint32_t s_cnt[];
     const KeyTile t = tiles[blockIdx.x];
+    const uint64_t *__restrict__ keys = t.keys;
     if (PRIV) {
         for (uint32_t i = threadIdx.x; i < t.nb; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
@@ -133,9 +135,10 @@ __global__ __launch_bounds__(256) void k_bucket_hist(const uint64_t *__restrict_
     }
 }
 
-// Per bucket: node count (memo), size histogram, oversize flag.
+// Per bucket: node count (tree_nodes table by bucket size), size histogram,
+// oversize flag.
 __global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict__ counts, uint64_t nb,
-                                                     const uint32_t *__restrict__ memo, uint64_t *__restrict__ counts64,
+                                                     const uint32_t *__restrict__ tree_nodes, uint64_t *__restrict__ counts64,
                                                      uint64_t *__restrict__ nodes, uint32_t *__restrict__ size_hist) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
@@ -147,13 +150,13 @@ __global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict_
         return;
     }
     atomicAdd(&size_hist[m], 1u);
-    nodes[b] = (memo[m] >> 16) & 0x7FF;
+    nodes[b] = tree_nodes[m];
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const uint64_t *__restrict__ keys, const KeyTile *__restrict__ tiles,
-                                                 const uint64_t *__restrict__ bucket_acc, uint32_t *__restrict__ cursor,
-                                                 uint64_t *__restrict__ sorted) {
+__global__ __launch_bounds__(256) void k_scatter(const KeyTile *__restrict__ tiles, const uint64_t *__restrict__ bucket_acc,
+                                                 uint32_t *__restrict__ cursor, uint64_t *__restrict__ sorted) {
     const KeyTile t = tiles[blockIdx.x];
+    const uint64_t *__restrict__ keys = t.keys;
     for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) {
         const uint64_t b = t.bbase + __umul64hi(keys[2 * i], t.nb);
         const uint32_t p = atomicAdd(&cursor[b], 1u);
@@ -336,23 +339,104 @@ __device__ __forceinline__ void stream_keys(const uint64_t *__restrict__ kp, uin
     if (i + 2 < end) f(n2);
 }
 
+// Scalar loads of wave-uniform keys into scalar registers.  Each asm block waits
+// for its own load (s_waitcnt in the same block), so no load is in flight when the
+// block ends and every output register holds its final value.
+__device__ __forceinline__ void sload_keys8(const uint64_t *p, uint64_t (&k)[8]) {
+    typedef unsigned int u32x16 __attribute__((ext_vector_type(16)));
+    u32x16 r;
+    __asm__ volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=&s"(r) : "s"(p) : "memory");
+#pragma unroll
+    for (int j = 0; j < 8; j++) k[j] = ((uint64_t)r[2 * j + 1] << 32) | r[2 * j];
+}
+__device__ __forceinline__ uint64_t sload_key(const uint64_t *p) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 r;
+    __asm__ volatile("s_load_dwordx2 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=&s"(r) : "s"(p) : "memory");
+    return ((uint64_t)r[1] << 32) | r[0];
+}
+
+// Streams keys kp[begin, end) (kp wave-uniform) through scalar registers, eight
+// keys per scalar load.  The wait of each load also waits for the LDS reads of
+// the aggregation table issued before it (both share one counter), so loads are
+// not issued ahead: other waves on the SIMD cover their latency.
+template <typename F>
+__device__ __forceinline__ void stream_keys_scalar(const uint64_t *__restrict__ kp_any, uint32_t begin_any,
+                                                   uint32_t end_any, F &&f) {
+    // wave-uniform by construction; readfirstlane lets the compiler keep them
+    // (and the loop) in scalar registers
+    const uint64_t *kp = (const uint64_t *)uni64((uint64_t)kp_any);
+    const uint32_t end = uni(end_any);
+    uint32_t i = uni(begin_any);
+    for (; i + 8 <= end; i += 8) {
+        uint64_t k[8];
+        sload_keys8(kp + i, k);
+#pragma unroll
+        for (int j = 0; j < 8; j++) f(k[j]);
+    }
+    for (; i < end; i++) f(sload_key(kp + i));
+}
+
+// Candidate seeds of one group: lane-u candidate = base + off[u] (mod 2^64), base
+// wave-uniform, off[u] < 2^32 per lane, max_off >= every off[u].
+template <int U>
+struct Seeds {
+    uint64_t base;
+    uint32_t off[U];
+    uint32_t max_off;
+    // candidate j = u * 64 + lane sits at base + j * stride
+    __device__ __forceinline__ Seeds(uint64_t b, uint32_t stride) : base(b) {
+        const uint32_t lane = lane_id();
+#pragma unroll
+        for (int u = 0; u < U; u++) off[u] = (u * 64 + lane) * stride;
+        max_off = (U * 64 - 1) * stride;
+    }
+};
+
+// Calls g(u, h16) for every key of kp[begin, end) and every candidate u, where
+// h16 = fold_h16(key + base + off[u]) (rs_common.h).  SK: keys in scalar
+// registers; key + base is then one scalar 64-bit add, and while its low word
+// plus max_off does not carry, key + base + off[u] has that sum as low word and
+// the uniform high word, so each candidate costs one 32-bit add instead of a
+// 64-bit one.  Otherwise (rare: within max_off of a 2^32 boundary) the full
+// 64-bit add.  Without SK: vector key loads and per-lane 64-bit seeds.
+template <int U, bool SK, typename G>
+__device__ __forceinline__ void each_hash(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end,
+                                          const Seeds<U> &sd, G &&g) {
+    if (SK) {
+        const uint64_t base = uni64(sd.base);
+        const uint32_t max_off = uni(sd.max_off);
+        stream_keys_scalar(kp, begin, end, [&](uint64_t k) {
+            const uint64_t S = k + base;
+            const uint32_t lo = (uint32_t)S, hi = (uint32_t)(S >> 32);
+            if (lo <= ~max_off) {
+#pragma unroll
+                for (int u = 0; u < U; u++) g(u, h16_of(hash_pre32((lo + sd.off[u]) ^ hi)));
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) g(u, search_h16(S + sd.off[u]));
+            }
+        });
+    } else {
+        uint64_t s[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) s[u] = sd.base + sd.off[u];
+        stream_keys(kp, begin, end, [&](uint64_t k) {
+#pragma unroll
+            for (int u = 0; u < U; u++) g(u, search_h16(k + s[u]));
+        });
+    }
+}
+
 // Fanout-2 node: lowest x with #{remap(k+S[depth]+x, m) < split} == split
 // (SIMDRecSplit.hpp:250-317).  remap < split <=> h16 < ceil(split*2^16/m).
-template <int U>
+template <int U, bool SK>
 __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                  uint32_t split, uint32_t H, uint64_t u0) {
-    const uint32_t lane = lane_id();
-    uint64_t s[U];
     uint32_t cnt[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane) + sconst;
-        cnt[u] = 0;
-    }
-    stream_keys(kp, 0, m, [&](uint64_t k) {
-#pragma unroll
-        for (int u = 0; u < U; u++) cnt[u] += search_h16(k + s[u]) < H;
-    });
+    for (int u = 0; u < U; u++) cnt[u] = 0;
+    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += h < H; });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(cnt[u] == split);
@@ -367,29 +451,20 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
 // forces another count negative, so equality of the counted fields is
 // equality of all counts.  With LUT, the increment 1 << (fb * part) of every
 // remapped value r < m is read from a per-wave LDS table instead of computed.
-template <int U, bool WIDE, bool LUT>
+template <int U, bool WIDE, bool LUT, bool SK>
 __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask,
                                                uint32_t lut_hi, uint32_t two, uint64_t u0) {
     using C = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
-    const uint32_t lane = lane_id();
-    uint64_t s[U];
     C cnt[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane) + sconst;
-        cnt[u] = 0;
-    }
-    stream_keys(kp, 0, m, [&](uint64_t k) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t h = search_h16(k + s[u]);
-            if (LUT) {
-                cnt[u] += *(const __attribute__((address_space(3))) uint32_t *)(uintptr_t)lut_byte_addr(h, m, lut_hi, two);
-            } else {
-                const uint32_t part = __umul24(__umul24(h, m) >> 16, magic) >> 20;
-                cnt[u] += (C)1 << __umul24(part, fb);
-            }
+    for (int u = 0; u < U; u++) cnt[u] = 0;
+    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) {
+        if (LUT) {
+            cnt[u] += *(const __attribute__((address_space(3))) uint32_t *)(uintptr_t)lut_byte_addr(h, m, lut_hi, two);
+        } else {
+            const uint32_t part = __umul24(__umul24(h, m) >> 16, magic) >> 20;
+            cnt[u] += (C)1 << __umul24(part, fb);
         }
     });
 #pragma unroll
@@ -404,27 +479,17 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
 // The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
-template <int U>
+template <int U, bool SK>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
                                                    uint32_t nr, uint64_t u0) {
-    const uint32_t lane = lane_id();
     const uint64_t S = c_start_seed[kSeedLeaf];
     const uint32_t full = (uint32_t)((1ull << L) - 1);
-    uint64_t sb[U];
     uint32_t ml[U], mr[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        sb[u] = (u0 + (uint64_t)(u * 64 + lane)) * L + S;
-        ml[u] = mr[u] = 0;
-    }
-    stream_keys(kp, 0, nl, [&](uint64_t k) {
-#pragma unroll
-        for (int u = 0; u < U; u++) ml[u] |= 1u << (__umul24(search_h16(k + sb[u]), L) >> 16);
-    });
-    stream_keys(kp, nl, L, [&](uint64_t k) {
-#pragma unroll
-        for (int u = 0; u < U; u++) mr[u] |= 1u << (__umul24(search_h16(k + sb[u]), L) >> 16);
-    });
+    for (int u = 0; u < U; u++) ml[u] = mr[u] = 0;
+    const Seeds<U> sd(u0 * L + S, L);
+    each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
+    each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
@@ -452,32 +517,22 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
 // Leaf without rotation (SIMDRecSplit.hpp:510-599): lowest x whose m keys land on
 // m distinct positions.  A prefix of ceil(3 sqrt(m)) keys must already be
 // collision-free; waves skip the rest when no lane passes (lossless).
-template <int U>
+template <int U, bool SK>
 __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict__ kp, uint32_t m, uint32_t mid,
                                                      uint64_t u0) {
-    const uint32_t lane = lane_id();
     const uint64_t S = c_start_seed[kSeedLeaf];
     const uint32_t full = (uint32_t)((1ull << m) - 1);
-    uint64_t s[U];
     uint32_t mk[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        s[u] = u0 + (uint64_t)(u * 64 + lane) + S;
-        mk[u] = 0;
-    }
-    stream_keys(kp, 0, mid, [&](uint64_t k) {
-#pragma unroll
-        for (int u = 0; u < U; u++) mk[u] |= 1u << (__umul24(search_h16(k + s[u]), m) >> 16);
-    });
+    for (int u = 0; u < U; u++) mk[u] = 0;
+    const Seeds<U> sd(u0 + S, 1);
+    each_hash<U, SK>(kp, 0, mid, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, m) >> 16); });
     if (mid < m) {
         bool any = false;
 #pragma unroll
         for (int u = 0; u < U; u++) any |= (uint32_t)__popc(mk[u]) == mid;
         if (!__ballot(any)) return kNone;
-        stream_keys(kp, mid, m, [&](uint64_t k) {
-#pragma unroll
-            for (int u = 0; u < U; u++) mk[u] |= 1u << (__umul24(search_h16(k + s[u]), m) >> 16);
-        });
+        each_hash<U, SK>(kp, mid, m, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, m) >> 16); });
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -493,7 +548,7 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-template <int KIND, int U, bool WIDE, bool LUT>
+template <int KIND, int U, bool WIDE, bool LUT, bool SK>
 __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
@@ -633,11 +688,11 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 if (u0 * scale >= r) break;
             }
             work += (uint64_t)U * 64 * m;
-            if (KIND == kHigher) hit = group_higher<U>(kp, m, sconst, aux0, aux1, u0);
+            if (KIND == kHigher) hit = group_higher<U, SK>(kp, m, sconst, aux0, aux1, u0);
             else if (KIND == kUpper || KIND == kLower)
-                hit = group_aggr<U, WIDE, LUT>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two, u0);
-            else if (KIND == kLeafRot) hit = group_leaf_rot<U>(kp, m, aux0, aux1, u0);
-            else hit = group_leaf_brute<U>(kp, m, aux0, u0);
+                hit = group_aggr<U, WIDE, LUT, SK>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two, u0);
+            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK>(kp, m, aux0, aux1, u0);
+            else hit = group_leaf_brute<U, SK>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
         if (hit != kNone && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)hit);
@@ -755,6 +810,7 @@ struct EncodeArgs {
     const uint64_t *seeds;
     const uint16_t *node_m;
     const uint32_t *memo;
+    const uint32_t *tree_nodes;  // nodes of a bucket by size (unmasked)
     uint64_t nb;
     SplitParams P;
     int rotate;
@@ -779,7 +835,7 @@ __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__r
         return;
     }
     const uint64_t base = a.node_base[b];
-    const uint32_t nn = (a.memo[m] >> 16) & 0x7FF;
+    const uint32_t nn = a.tree_nodes[m];
     uint64_t un = 0;
     uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0};
     for (uint32_t j = lane; j < nn; j += 64) {
@@ -833,7 +889,7 @@ __global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t
     const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
     if (m < 2) return;
     const uint64_t base = a.node_base[b];
-    const uint32_t nn = (a.memo[m] >> 16) & 0x7FF;
+    const uint32_t nn = a.tree_nodes[m];
     const uint64_t p0 = pos[b] + (uint64_t)bit_off[b];
     const uint64_t fixed_total = a.memo[m] & 0xFFFF;
     uint64_t fpre = 0, upre = 0;
@@ -991,14 +1047,22 @@ struct QueryArgs {
     const uint32_t *memo;
     uint64_t nbuckets, bpk, lc, lp;
     long long min_delta, min_diff;
+    // readable words of each array; reads beyond see all-ones words, so a
+    // malformed deserialized structure cannot drive a scan out of its buffer
+    uint64_t n_rice, n_lower, n_uc, n_up, n_jump;
+    uint64_t keys_count;
     SplitParams P;
     int rotate;
 };
 
-__device__ __forceinline__ uint64_t ld_bytes64(const uint64_t *base, uint64_t byte) {
+__device__ __forceinline__ uint64_t rd_word(const uint64_t *base, uint64_t n, uint64_t i) {
+    return i < n ? base[i] : ~0ull;
+}
+
+__device__ __forceinline__ uint64_t ld_bytes64(const uint64_t *base, uint64_t n, uint64_t byte) {
     const uint64_t w = byte >> 3, sh = (byte & 7) * 8;
-    const uint64_t lo = base[w];
-    return sh ? (lo >> sh) | (base[w + 1] << (64 - sh)) : lo;
+    const uint64_t lo = rd_word(base, n, w);
+    return sh ? (lo >> sh) | (rd_word(base, n, w + 1) << (64 - sh)) : lo;
 }
 
 __device__ __forceinline__ uint64_t select64(uint64_t x, uint64_t k) {
@@ -1017,18 +1081,18 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
     const uint64_t lc = q.lc, lp = q.lp;
     const uint64_t mask_c = (1ull << lc) - 1, mask_p = (1ull << lp) - 1;
     const uint64_t pos_lower = i * (lc + lp);
-    uint64_t lower = ld_bytes64(q.lower, pos_lower / 8) >> (pos_lower % 8);
+    uint64_t lower = ld_bytes64(q.lower, q.n_lower, pos_lower / 8) >> (pos_lower % 8);
     const uint64_t jsq = (i >> 14) * kEfSuperSizeDev * 2;
     const uint64_t jin = (i & ((1 << 14) - 1)) >> 8;
-    const uint64_t pair = ld_bytes64(q.jump, (jsq + 2) * 8 + 4 * jin);
-    const uint64_t jump_c = q.jump[jsq] + (pair & 0xFFFF);
-    const uint64_t jump_p = q.jump[jsq + 1] + ((pair >> 16) & 0xFFFF);
+    const uint64_t pair = ld_bytes64(q.jump, q.n_jump, (jsq + 2) * 8 + 4 * jin);
+    const uint64_t jump_c = rd_word(q.jump, q.n_jump, jsq) + (pair & 0xFFFF);
+    const uint64_t jump_p = rd_word(q.jump, q.n_jump, jsq + 1) + ((pair >> 16) & 0xFFFF);
     uint64_t wc = jump_c / 64, wp = jump_p / 64;
-    uint64_t win_c = q.upper_c[wc] & (~0ull << (jump_c % 64));
-    uint64_t win_p = q.upper_p[wp] & (~0ull << (jump_p % 64));
+    uint64_t win_c = rd_word(q.upper_c, q.n_uc, wc) & (~0ull << (jump_c % 64));
+    uint64_t win_p = rd_word(q.upper_p, q.n_up, wp) & (~0ull << (jump_p % 64));
     uint64_t dc = i & 255, dp = i & 255;
-    for (uint64_t c; (c = (uint64_t)__popcll(win_c)) <= dc; dc -= c) win_c = q.upper_c[++wc];
-    for (uint64_t c; (c = (uint64_t)__popcll(win_p)) <= dp; dp -= c) win_p = q.upper_p[++wp];
+    for (uint64_t c; (c = (uint64_t)__popcll(win_c)) <= dc; dc -= c) win_c = rd_word(q.upper_c, q.n_uc, ++wc);
+    for (uint64_t c; (c = (uint64_t)__popcll(win_p)) <= dp; dp -= c) win_p = rd_word(q.upper_p, q.n_up, ++wp);
     const uint64_t sel_c = select64(win_c, dc);
     const uint64_t cum_delta = i * (uint64_t)q.min_delta;
     uint64_t cum_keys = ((wc * 64 + sel_c - i) << lc | (lower & mask_c)) + cum_delta;
@@ -1036,26 +1100,30 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
     const uint64_t bit_pos = ((wp * 64 + select64(win_p, dp) - i) << lp | (lower & mask_p)) + i * (uint64_t)q.min_diff +
                              (uint64_t)(long long)(q.bpk * cum_keys >> 20);
     win_c &= (~0ull << sel_c) << 1;
-    while (win_c == 0) win_c = q.upper_c[++wc];
+    while (win_c == 0) win_c = rd_word(q.upper_c, q.n_uc, ++wc);
     lower >>= lp;
     const uint64_t cum_next = ((wc * 64 + (uint64_t)__ffsll((long long)win_c) - 1 - i - 1) << lc | (lower & mask_c)) +
                               cum_delta + (uint64_t)q.min_delta;
     uint64_t m = cum_next - cum_keys;
+    if (m >= kMaxBucket) {  // malformed structure: no valid answer
+        out[t] = q.keys_count;
+        return;
+    }
     // Golomb-Rice reader (RiceBitVector.hpp:155-218)
     uint64_t fixed_off = bit_pos;
     uint64_t up = bit_pos + (q.memo[m] & 0xFFFF);
     uint64_t uw = up / 64;
-    uint64_t window = q.rice[uw++] >> (up & 63);
+    uint64_t window = rd_word(q.rice, q.n_rice, uw++) >> (up & 63);
     int valid = 64 - (int)(up & 63);
     auto read_next = [&](uint32_t g) -> uint64_t {
         uint64_t result = 0;
         if (window == 0) {
             result += (uint64_t)valid;
-            window = q.rice[uw++];
+            window = rd_word(q.rice, q.n_rice, uw++);
             valid = 64;
             while (window == 0) {
                 result += 64;
-                window = q.rice[uw++];
+                window = rd_word(q.rice, q.n_rice, uw++);
             }
         }
         const uint64_t p = (uint64_t)__ffsll((long long)window) - 1;
@@ -1063,14 +1131,14 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
         window >>= 1;
         valid -= (int)p + 1;
         result = (result + p) << g;
-        result |= (ld_bytes64(q.rice, fixed_off / 8) >> (fixed_off % 8)) & ((1ull << g) - 1);
+        result |= (ld_bytes64(q.rice, q.n_rice, fixed_off / 8) >> (fixed_off % 8)) & ((1ull << g) - 1);
         fixed_off += g;
         return result;
     };
     auto skip = [&](uint64_t nodes, uint64_t flen) {
         uint64_t missing = nodes, c;
         while ((c = (uint64_t)__popcll(window)) < missing) {
-            window = q.rice[uw++];
+            window = rd_word(q.rice, q.n_rice, uw++);
             missing -= c;
             valid = 64;
         }

@@ -1,39 +1,153 @@
-"""Served construction: a request loop with continuous batching on one GPU.
+"""Served construction: Python mirror of the library's construction server.
 
-Jobs (key set, leaf size, bucket size) arrive over time.  Worker threads, each
-owning a build context (its own HIP stream), take every pending job that shares
-the oldest pending job's (leaf, bucket) and build them together with one
-rs_build_batch call; builds of different workers overlap on the device.  A job
-is admitted only while the estimated HBM footprint of everything admitted stays
-under the budget; it stays queued otherwise.  Each job records arrival, start and
-completion time; its MPHF is the same as a standalone build of its keys
-(rs_build_batch builds every set independently).
+The server (rs_server_* in include/gpurecsplit_amd.h, gpurecsplit_amd/csrc/
+rs_server.cpp) accepts jobs of differing key count and leaf size at any time,
+copies each job's keys into device memory on arrival, and builds ready jobs with
+continuous batching: `slots` build contexts (one HIP stream each) take the oldest
+ready job with every ready job of the same (leaf, bucket, rotation) and build them
+as one batch, admitted only while the key pool, the slots' buffers and the batch's
+working-set bound fit the HBM budget.  Each job's MPHF equals its standalone build;
+a failing batch is retried job by job so only the failing job reports its error.
+
+    with Server(slots=4) as srv:
+        jid = srv.submit(keys, leaf=16, bucket=2000)
+        rs, info = srv.wait(jid)           # GPURecSplit, timing of the job
 """
 from __future__ import annotations
 
-import threading
+import ctypes as C
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Context, GPURecSplit, build_batch
-from .keys import splitmix64_keys
+from . import GPURecSplit, RecSplitError, _check, _keys_array, lib
 
-# HBM bytes per key of one build: input keys 16 B, bucket-sorted copy 8 B, plus
-# node descriptors, seeds and search state (~1 node per 4-8 keys, 58 B each).
-BYTES_PER_KEY = 40
+__all__ = ["Server", "Job", "poisson_stream", "run_stream"]
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+RS_ERR_PENDING = 12
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("max_batch_keys", C.c_uint64), ("time_limit_s", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("status", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double), ("n", C.c_uint64), ("leaf", C.c_uint32), ("bucket", C.c_uint64),
+                ("batch_id", C.c_uint64), ("batch_jobs", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"state": JOB_STATES.get(self.state, self.state), "status": self.status, "t_submit": self.t_submit,
+                "t_start": self.t_start, "t_done": self.t_done, "n": int(self.n), "leaf": int(self.leaf),
+                "bucket": int(self.bucket), "batch_id": int(self.batch_id), "batch_jobs": int(self.batch_jobs)}
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(name, C.c_uint64) for name in (
+        "jobs_submitted", "jobs_done", "jobs_failed", "keys_done", "batches", "batches_split", "committed_bytes",
+        "peak_committed_bytes", "key_pool_bytes", "key_pool_used_bytes", "jobs_waiting_for_memory", "jobs_ready")
+    ] + [("uptime_s", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_vp, _u64p = C.c_void_p, C.POINTER(C.c_uint64)
+for _name, (_res, _args) in {
+    "rs_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "rs_server_destroy": (None, [_vp]),
+    "rs_submit_job": (C.c_int, [_vp, _u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, _u64p]),
+    "rs_submit_job_device": (C.c_int, [_vp, C.c_void_p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, _u64p]),
+    "rs_submit_batch": (C.c_int, [_vp, _u64p, _u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, _u64p]),
+    "rs_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(JobInfo)]),
+    "rs_wait": (C.c_int, [_vp, C.c_uint64, C.c_double, C.POINTER(_vp), C.POINTER(JobInfo)]),
+    "rs_server_stats": (C.c_int, [_vp, C.POINTER(ServeStats)]),
+}.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class Server:
+    """Construction server on one GPU (rs_server_create)."""
+
+    def __init__(self, slots: int = 4, hbm_budget_bytes: int = 64 << 30, max_batch_keys: int = 50_000_000,
+                 device: int = 0, time_limit_s: float = 0.0):
+        cfg = ServerConfig(device, slots, hbm_budget_bytes, max_batch_keys, time_limit_s)
+        h = C.c_void_p()
+        _check(lib.rs_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def submit(self, keys, leaf: int, bucket: int, rotate: bool = True) -> int:
+        """Submits one job; keys: (n, 2) uint64 host array (copied at once) or a
+        contiguous (n, 2) 64-bit CUDA tensor (used in place: keep it alive until wait)."""
+        jid = C.c_uint64()
+        if hasattr(keys, "data_ptr") and getattr(keys, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream(keys.device).synchronize()
+            _check(lib.rs_submit_job_device(self.h, C.c_void_p(keys.data_ptr()), keys.shape[0], leaf, bucket,
+                                            int(rotate), C.byref(jid)))
+        else:
+            k = _keys_array(keys)
+            _check(lib.rs_submit_job(self.h, k.ctypes.data_as(_u64p), k.shape[0], leaf, bucket, int(rotate),
+                                     C.byref(jid)))
+        return int(jid.value)
+
+    def poll(self, job_id: int) -> dict:
+        info = JobInfo()
+        _check(lib.rs_poll(self.h, job_id, C.byref(info)))
+        return info.as_dict()
+
+    def wait(self, job_id: int, timeout_s: float = -1.0):
+        """(GPURecSplit, info dict) of a finished job; raises RecSplitError for a
+        failed job, and TimeoutError if it is still running after timeout_s."""
+        info = JobInfo()
+        out = C.c_void_p()
+        rc = lib.rs_wait(self.h, job_id, timeout_s, C.byref(out), C.byref(info))
+        if rc == RS_ERR_PENDING:
+            raise TimeoutError(f"job {job_id} still {JOB_STATES.get(info.state)}")
+        d = info.as_dict()
+        if rc != 0:
+            err = RecSplitError(rc, (lib.rs_last_error() or b"").decode())
+            err.info = d
+            raise err
+        rs = GPURecSplit(bucket_size=int(info.bucket), leaf_size=int(info.leaf), _handle=out)
+        return rs, d
+
+    def stats(self) -> dict:
+        s = ServeStats()
+        _check(lib.rs_server_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self, _destroy=lib.rs_server_destroy):
+        if getattr(self, "h", None):
+            _destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 @dataclass
 class Job:
     job_id: int
-    keys: np.ndarray          # (n, 2) uint64
+    keys: np.ndarray
     leaf: int
     bucket: int
-    arrival: float            # seconds on the server clock
+    arrival: float
     start: float = -1.0
     done: float = -1.0
+    batch_jobs: int = 0
     result: GPURecSplit | None = None
     error: str | None = None
 
@@ -47,127 +161,25 @@ class Job:
 
 
 @dataclass
-class ServerStats:
+class StreamSummary:
     jobs: int = 0
     keys: int = 0
-    batches: int = 0
     wall_s: float = 0.0
     latencies: list = field(default_factory=list)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
-        return {"jobs": self.jobs, "keys": self.keys, "batches": self.batches, "wall_s": self.wall_s,
+        return {"jobs": self.jobs, "keys": self.keys, "wall_s": self.wall_s,
                 "objects_per_s": self.keys / self.wall_s if self.wall_s else 0.0,
                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                 "mean_latency_s": float(lat.mean())}
 
 
-class Server:
-    """Continuous-batching construction server over `workers` concurrent contexts."""
-
-    def __init__(self, workers: int = 2, hbm_budget_bytes: int = 64 << 30, max_batch_keys: int = 50_000_000,
-                 device: int = 0, time_limit_s: float | None = None, on_batch=None):
-        """time_limit_s bounds each batch build (its jobs fail with the time-limit
-        error past it); on_batch(jobs) is called after every finished batch."""
-        import torch
-
-        self.torch = torch
-        self.device = device
-        self.on_batch = on_batch
-        self.ctxs = [Context(device, time_limit_s=time_limit_s) for _ in range(workers)]
-        self.budget = hbm_budget_bytes
-        self.max_batch_keys = max_batch_keys
-        self.pending: list[Job] = []
-        self.admitted_bytes = 0
-        self.cv = threading.Condition()
-        self.closed = False
-        self.t0 = time.perf_counter()
-        self.stats = ServerStats()
-        self.threads = [threading.Thread(target=self._worker, args=(c,), daemon=True) for c in self.ctxs]
-        for t in self.threads:
-            t.start()
-
-    def now(self) -> float:
-        return time.perf_counter() - self.t0
-
-    def submit(self, job: Job):
-        with self.cv:
-            self.pending.append(job)
-            self.cv.notify_all()
-
-    def close(self):
-        with self.cv:
-            self.closed = True
-            self.cv.notify_all()
-        for t in self.threads:
-            t.join()
-        self.stats.wall_s = self.now()
-
-    def _take(self) -> list[Job] | None:
-        """Called with the lock held: the next admissible batch, or None."""
-        if not self.pending:
-            return None
-        head = self.pending[0]
-        group, keys, need = [], 0, 0
-        for j in self.pending:
-            if (j.leaf, j.bucket) != (head.leaf, head.bucket):
-                continue
-            b = j.n * BYTES_PER_KEY
-            if group and (keys + j.n > self.max_batch_keys or self.admitted_bytes + need + b > self.budget):
-                break
-            if not group and self.admitted_bytes + b > self.budget and self.admitted_bytes > 0:
-                return None  # wait for memory to free up
-            group.append(j)
-            keys += j.n
-            need += b
-        for j in group:
-            self.pending.remove(j)
-        self.admitted_bytes += need
-        return group
-
-    def _worker(self, ctx: Context):
-        torch = self.torch
-        while True:
-            with self.cv:
-                batch = None
-                while batch is None:
-                    batch = self._take()
-                    if batch is None:
-                        if self.closed and not self.pending:
-                            return
-                        self.cv.wait(timeout=0.05)
-            start = self.now()
-            for j in batch:
-                j.start = start
-            try:
-                keys = np.concatenate([j.keys for j in batch])
-                off = np.concatenate([[0], np.cumsum([j.n for j in batch])]).astype(np.uint64)
-                d = torch.from_numpy(keys.view(np.int64)).to(f"cuda:{self.device}")
-                results, _ = build_batch(d, off, batch[0].bucket, leaf_size=batch[0].leaf, ctx=ctx)
-                del d
-                for j, r in zip(batch, results):
-                    j.result = r
-            except Exception as e:  # the whole batch fails together
-                for j in batch:
-                    j.error = str(e)
-            done = self.now()
-            with self.cv:
-                for j in batch:
-                    j.done = done
-                    self.stats.jobs += 1
-                    self.stats.keys += j.n
-                    self.stats.latencies.append(j.latency)
-                self.stats.batches += 1
-                self.admitted_bytes -= sum(j.n * BYTES_PER_KEY for j in batch)
-                self.cv.notify_all()
-            if self.on_batch is not None:
-                self.on_batch(batch)
-
-
 def poisson_stream(num_jobs: int, rate_per_s: float, seed: int, size_lo: int, size_hi: int,
                    leaf_weights: dict[int, float], bucket_of_leaf, size_cap_of_leaf=None):
     """Synthetic job stream: Poisson arrivals, log-uniform sizes, leaf sizes drawn
-    with the given weights; keys are splitmix64 draws (first, second)."""
+    with the given weights; each job's keys are a splitmix64 stream of its own seed.
+    Returns (arrival s, n, leaf, bucket, key seed) per job."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate_per_s, size=num_jobs))
     sizes = np.exp(rng.uniform(np.log(size_lo), np.log(size_hi), size=num_jobs)).astype(np.int64)
@@ -182,19 +194,46 @@ def poisson_stream(num_jobs: int, rate_per_s: float, seed: int, size_lo: int, si
     return jobs
 
 
-def run_stream(stream, workers: int = 2, hbm_budget_bytes: int = 64 << 30, time_limit_s: float | None = None,
-               on_batch=None) -> tuple[list[Job], dict]:
-    """Replays a stream in real time against a Server and returns (jobs, summary).
-    Keys are generated before the server clock starts."""
-    prepared = [(t_arr, splitmix64_keys(kseed, n), leaf, bucket) for (t_arr, n, leaf, bucket, kseed) in stream]
-    srv = Server(workers=workers, hbm_budget_bytes=hbm_budget_bytes, time_limit_s=time_limit_s, on_batch=on_batch)
-    jobs = []
-    for i, (t_arr, keys, leaf, bucket) in enumerate(prepared):
-        delay = t_arr - srv.now()
-        if delay > 0:
-            time.sleep(delay)
-        j = Job(i, keys, leaf, bucket, arrival=srv.now())
-        jobs.append(j)
-        srv.submit(j)
-    srv.close()
-    return jobs, srv.stats.summary()
+def run_stream(stream, slots: int = 4, hbm_budget_bytes: int = 64 << 30, time_limit_s: float = 0.0,
+               on_done=None, keys_of=None) -> tuple[list[Job], dict]:
+    """Replays a stream in real time against a Server: each job is submitted at
+    its arrival time (keys generated beforehand), results are collected as they
+    finish.  Returns (jobs, summary with p50/p99 latency = done - arrival and
+    objects/s over the stream's wall time).  keys_of(job tuple) may supply keys."""
+    from .keys import splitmix64_keys
+
+    prepared = [(s[0], keys_of(s) if keys_of else splitmix64_keys(s[4], s[1]), s[2], s[3]) for s in stream]
+    summ = StreamSummary()
+    jobs: list[Job] = []
+    with Server(slots=slots, hbm_budget_bytes=hbm_budget_bytes, time_limit_s=time_limit_s) as srv:
+        t0 = time.perf_counter()
+        pending = []
+        for i, (t_arr, keys, leaf, bucket) in enumerate(prepared):
+            delay = t_arr - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            jid = srv.submit(keys, leaf, bucket)
+            j = Job(jid, keys, leaf, bucket, arrival=time.perf_counter() - t0)
+            jobs.append(j)
+            pending.append(j)
+        for j in pending:
+            try:
+                rs, info = srv.wait(j.job_id)
+                j.result = rs
+            except RecSplitError as e:
+                info = e.info
+                j.error = str(e)
+            # the server clock starts at Server creation, slightly before t0
+            skew = info["t_submit"] - j.arrival
+            j.start, j.done = info["t_start"] - skew, info["t_done"] - skew
+            j.batch_jobs = info["batch_jobs"]
+            summ.jobs += 1
+            summ.keys += j.n
+            summ.latencies.append(j.latency)
+            if on_done is not None:
+                on_done(j)
+        summ.wall_s = max(j.done for j in jobs) if jobs else 0.0
+        out = summ.summary()
+        out["server"] = srv.stats()
+    out["batches"] = out["server"]["batches"]
+    return jobs, out

@@ -63,8 +63,10 @@ enum rs_status {
     RS_ERR_EF_OVERFLOW = 6,       /* Elias-Fano jump offset >= 2^16 (reference aborts) */
     RS_ERR_NOMEM = 7,             /* host or device allocation failed */
     RS_ERR_FORMAT = 8,            /* malformed serialized stream or leaf-size mismatch */
-    RS_ERR_NO_DEVICE = 9,         /* handle has no device copy (deserialized) */
-    RS_ERR_TIMEOUT = 10           /* the build passed the context's time limit */
+    RS_ERR_NO_DEVICE = 9,         /* reserved (no longer returned: device copies are made on demand) */
+    RS_ERR_TIMEOUT = 10,          /* the build passed the context's time limit */
+    RS_ERR_INTERNAL = 11,         /* an internal consistency check failed */
+    RS_ERR_PENDING = 12           /* rs_wait: the job did not finish within the timeout */
 };
 
 /* Per-build measurements (HIP events on the build stream). */
@@ -119,6 +121,85 @@ int rs_build_device(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf
 int rs_build_batch(rs_ctx *ctx, const uint64_t *d_keys, const uint64_t *set_offsets, size_t num_sets,
                    uint32_t leaf_size, size_t bucket_size, int rotate, rs_mphf **out, rs_build_stats *stats);
 
+/* Device memory the context holds for reuse by later builds (grow-only buffers),
+ * and their release.  No reference counterpart. */
+uint64_t rs_ctx_device_bytes(rs_ctx *ctx);
+int rs_ctx_release(rs_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Construction server: asynchronous jobs with continuous batching on one GPU.
+ * No reference counterpart (the reference builds one MPHF per synchronous
+ * constructor call, GPURecSplit.cuh:484-489).
+ *
+ * A job's keys are copied into the server's device key pool when it is submitted
+ * (or, while the HBM budget is exhausted, kept in host memory and uploaded as
+ * memory frees up).  `slots` build contexts, each with its own HIP stream, take
+ * the oldest ready job together with every ready job of the same (leaf size,
+ * bucket size, rotation), up to max_batch_keys, and build them as one batch;
+ * jobs that arrive while batches run are taken by the next free slot, and the
+ * batches of different slots overlap on the device.  A batch starts only when
+ * the key pool, every slot's buffers and the batch's worst-case working set fit
+ * hbm_budget_bytes.  Every result equals the standalone build of its keys; a
+ * failing batch is retried job by job, so only the failing job reports an error.
+ * ------------------------------------------------------------------------- */
+typedef struct rs_server rs_server;
+
+typedef struct rs_server_config {
+    int device;                /* HIP device ordinal */
+    uint32_t slots;            /* concurrent batch builds; 0: 4 */
+    uint64_t hbm_budget_bytes; /* device bytes for keys and build buffers; 0: 64 GiB */
+    uint64_t max_batch_keys;   /* keys per batch; 0: 50,000,000 */
+    double time_limit_s;       /* per batch build (rs_ctx_set_time_limit); 0: none */
+} rs_server_config;
+
+enum rs_job_state { RS_JOB_QUEUED = 0, RS_JOB_RUNNING = 1, RS_JOB_DONE = 2, RS_JOB_FAILED = 3 };
+
+typedef struct rs_job_info {
+    int state;          /* rs_job_state */
+    int status;         /* rs_status of a finished job */
+    double t_submit;    /* seconds on the server clock (from rs_server_create) */
+    double t_start;     /* -1 until the job's batch starts */
+    double t_done;      /* -1 until the job finishes */
+    uint64_t n;
+    uint32_t leaf;
+    uint64_t bucket;
+    uint64_t batch_id;  /* batch the job was built in (0: not started) */
+    uint32_t batch_jobs;
+} rs_job_info;
+
+typedef struct rs_serve_stats {
+    uint64_t jobs_submitted, jobs_done, jobs_failed, keys_done;
+    uint64_t batches;           /* batch builds started */
+    uint64_t batches_split;     /* failed batches retried job by job */
+    uint64_t committed_bytes;   /* key pool + slot buffers + growth bound of running batches */
+    uint64_t peak_committed_bytes;
+    uint64_t key_pool_bytes, key_pool_used_bytes;
+    uint64_t jobs_waiting_for_memory, jobs_ready;
+    double uptime_s;
+} rs_serve_stats;
+
+int rs_server_create(const rs_server_config *cfg, rs_server **out);
+/* Finishes every submitted job, then frees the server (results not taken are freed). */
+void rs_server_destroy(rs_server *s);
+/* Host keys (n pairs); the call copies them, so the buffer may be reused at once. */
+int rs_submit_job(rs_server *s, const uint64_t *keys, size_t n, uint32_t leaf_size, size_t bucket_size, int rotate,
+                  uint64_t *job_id);
+/* Keys already in device memory of the server's GPU; they are used in place and
+ * must stay valid until rs_wait returns for the job. */
+int rs_submit_job_device(rs_server *s, const uint64_t *d_keys, size_t n, uint32_t leaf_size, size_t bucket_size,
+                         int rotate, uint64_t *job_id);
+/* num_sets host key sets (as rs_build_batch) submitted as separate jobs. */
+int rs_submit_batch(rs_server *s, const uint64_t *keys, const uint64_t *set_offsets, size_t num_sets,
+                    uint32_t leaf_size, size_t bucket_size, int rotate, uint64_t *job_ids);
+/* State of a job (not blocking). */
+int rs_poll(rs_server *s, uint64_t job_id, rs_job_info *info);
+/* Waits up to timeout_s seconds (< 0: no limit) for the job.  On success the
+ * job's MPHF is handed to the caller (*out, free with rs_mphf_free) and the job
+ * is forgotten; a failed job returns its status and is forgotten; a job still
+ * running returns RS_ERR_PENDING and stays. */
+int rs_wait(rs_server *s, uint64_t job_id, double timeout_s, rs_mphf **out, rs_job_info *info);
+int rs_server_stats(rs_server *s, rs_serve_stats *out);
+
 /* Serialization in the reference's stream layout. */
 size_t rs_mphf_serialized_size(const rs_mphf *h);
 int rs_mphf_serialize(const rs_mphf *h, uint8_t *buf, size_t cap);
@@ -129,7 +210,9 @@ uint64_t rs_mphf_query(const rs_mphf *h, uint64_t first, uint64_t second);
 /* Host query of n keys. */
 void rs_mphf_query_batch(const rs_mphf *h, const uint64_t *keys, size_t n, uint64_t *out);
 /* Batched query on the GPU: d_keys (n pairs) and d_out (n uint64) in device memory
- * of ctx's GPU; h must come from a build on that context. */
+ * of ctx's GPU.  Any handle works: one without a copy on ctx's GPU (deserialized,
+ * batch results, built on another device) is uploaded on its first device query
+ * and keeps that copy until freed. */
 int rs_query_device(rs_ctx *ctx, const rs_mphf *h, const uint64_t *d_keys, size_t n, uint64_t *d_out);
 
 /* Space of the structure in bits (Elias-Fano + descriptors). */

@@ -25,6 +25,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <istream>
 #include <iterator>
 #include <mutex>
@@ -131,8 +132,14 @@ class GPURecSplit {
     /** Returns the value in [0, n) associated with the given 128-bit hash. */
     size_t operator()(const hash128_t &hash) { return (size_t)rs_mphf_query(h, hash.first, hash.second); }
 
+    /** Bytes of the reference's AbstractParallelRecSplit object, the sizeof term of its
+     *  getBits(): bucket_size, nbuckets, keys_count (3 words), the RiceBitVector (one
+     *  sux::util::Vector) and the DoubleEF (4 Vectors and 10 words), with a
+     *  sux::util::Vector taken as pointer + size + capacity (24 bytes). */
+    static constexpr size_t kReferenceObjectBytes = 3 * 8 + 24 + (4 * 24 + 10 * 8);
+
     /** Space of the structure in bits (AbstractParallelRecSplit.hpp:476-478). */
-    size_t getBits() { return (size_t)rs_mphf_bits(h) + 8 * sizeof(*this); }
+    size_t getBits() { return (size_t)rs_mphf_bits(h) + 8 * kReferenceObjectBytes; }
 
     /** Number of keys of the function. */
     size_t size() const { return h ? (size_t)rs_mphf_num_keys(h) : 0; }
@@ -153,9 +160,36 @@ class GPURecSplit {
         return os;
     }
 
-    /** Reads one serialized structure (the rest of the stream is consumed). */
+    /** Reads exactly one serialized structure (GPURecSplit.cuh:827-841): the three
+     *  header words, the descriptor vector, the six Elias-Fano scalars and its four
+     *  vectors, each vector as a word count followed by the words (DoubleEF.hpp:115-135).
+     *  Bytes after the structure stay in the stream. */
     friend std::istream &operator>>(std::istream &is, GPURecSplit &rs) {
-        std::vector<uint8_t> buf((std::istreambuf_iterator<char>(is)), std::istreambuf_iterator<char>());
+        std::vector<uint8_t> buf;
+        auto take = [&](size_t bytes) {
+            const size_t at = buf.size();
+            buf.resize(at + bytes);
+            if (bytes && !is.read(reinterpret_cast<char *>(buf.data() + at), (std::streamsize)bytes))
+                throw gpurecsplit_amd::Error(RS_ERR_FORMAT, "operator>>: truncated stream");
+            return at;
+        };
+        auto word = [&]() {
+            uint64_t v;
+            std::memcpy(&v, buf.data() + take(8), 8);
+            return v;
+        };
+        auto vec = [&]() {
+            uint64_t n = word();
+            while (n) {  // in bounded chunks: a corrupt count fails at the end of the stream
+                const uint64_t k = n < (uint64_t(1) << 20) ? n : (uint64_t(1) << 20);
+                take(8 * (size_t)k);
+                n -= k;
+            }
+        };
+        for (int i = 0; i < 3; i++) word();
+        vec();
+        for (int i = 0; i < 6; i++) word();
+        for (int i = 0; i < 4; i++) vec();
         rs_mphf *nh = nullptr;
         gpurecsplit_amd::check(rs_mphf_deserialize(buf.data(), buf.size(), LEAF_SIZE, USE_BIJECTIONS_ROTATE ? 1 : 0, &nh),
                                "operator>>");

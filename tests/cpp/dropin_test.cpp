@@ -1,6 +1,9 @@
 // Exercises the C++ drop-in header (include/gpurecsplit_amd/GPURecSplit.hpp) the
 // way the reference's benchmark and tests use GPURecSplit:
 //   dropin_test load  <mphf.bin> <keys.bin> <leaf>   operator>> + operator() bijection (no GPU)
+//   dropin_test load2 <stream.bin> <keysA.bin> <keysB.bin> <leaf>
+//                                                   two structures back to back, then trailing
+//                                                   bytes: operator>> reads exactly one each (no GPU)
 //   dropin_test build <keys.bin> <leaf> <bucket> <out.bin>
 //                                                   constructor + operator() + operator<< (GPU)
 // keys.bin holds n little-endian (first, second) uint64 pairs.
@@ -51,6 +54,16 @@ static int run(int argc, char **argv) {
         std::printf("load ok: n=%zu bits/key=%.4f\n", keys.size(), (double)rs.getBits() / keys.size());
         return 0;
     }
+    if (mode == "load2") {
+        GPURecSplit<L> a, b;
+        std::ifstream in(argv[2], std::ios::binary);
+        in >> a >> b;
+        std::string rest((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        auto ka = read_keys(argv[3]), kb = read_keys(argv[4]);
+        if (check_bijection(a, ka) || check_bijection(b, kb)) return 1;
+        std::printf("load2 ok: n=%zu,%zu trailing=%s\n", ka.size(), kb.size(), rest.c_str());
+        return 0;
+    }
     auto keys = read_keys(argv[2]);
     const auto before = keys;
     GPURecSplit<L> rs(keys, std::stoul(argv[4]));
@@ -71,7 +84,8 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "usage: see source\n");
         return 2;
     }
-    const int leaf = std::stoi(std::string(argv[1]) == "load" ? argv[4] : argv[3]);
+    const std::string mode = argv[1];
+    const int leaf = std::stoi(mode == "load" ? argv[4] : mode == "load2" ? argv[5] : argv[3]);
     try {
         switch (leaf) {
         case 8: return run<8>(argc, argv);

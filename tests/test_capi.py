@@ -64,6 +64,80 @@ def test_deserialize_rejects_bad_streams():
     assert e.value.code == 8
 
 
+def _vec_offsets(data: bytes):
+    """Byte offsets of the length words of the five vectors of a serialized MPHF."""
+    import struct
+    offs, p = [], 24
+    for i in range(5):
+        offs.append(p)
+        n = struct.unpack_from("<Q", data, p)[0]
+        p += 8 + 8 * n
+        if i == 0:
+            p += 48  # the six Elias-Fano scalars follow the descriptor vector
+    assert p == len(data)
+    return offs
+
+
+def test_deserialize_rejects_inconsistent_headers():
+    import struct
+    keys = splitmix_keys(21, 5000)
+    data = OracleMPHF.build(keys, 8, 100).serialize()
+    rice_end = _vec_offsets(data)[1] - 48
+    cases = {
+        "key count": struct.pack("<Q", 7000),             # bucket count no longer matches the EF
+        "bucket size": struct.pack("<Q", 10),
+    }
+    for name, patch in cases.items():
+        off = 16 if name == "key count" else 8
+        bad = data[:off] + patch + data[off + 8:]
+        with pytest.raises(gra.RecSplitError) as e:
+            gra.GPURecSplit.deserialize(bad, 8)
+        assert e.value.code == 8, name
+    # EF bucket count changed
+    bad = data[:rice_end] + struct.pack("<Q", 49) + data[rice_end + 8:]
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit.deserialize(bad, 8)
+    assert e.value.code == 8
+    # one word cut from each EF vector (length word and data shortened together)
+    for vi in range(1, 5):
+        o = _vec_offsets(data)[vi]
+        n = struct.unpack_from("<Q", data, o)[0]
+        bad = data[:o] + struct.pack("<Q", n - 1) + data[o + 8:o + 8 + 8 * (n - 1)] + data[o + 8 + 8 * n:]
+        with pytest.raises(gra.RecSplitError) as e:
+            gra.GPURecSplit.deserialize(bad, 8)
+        assert e.value.code == 8, f"vector {vi}"
+    # empty descriptor vector
+    o = _vec_offsets(data)[0]
+    n = struct.unpack_from("<Q", data, o)[0]
+    bad = data[:o] + struct.pack("<Q", 0) + data[o + 8 + 8 * n:]
+    with pytest.raises(gra.RecSplitError) as e:
+        gra.GPURecSplit.deserialize(bad, 8)
+    assert e.value.code == 8
+
+
+def test_host_query_of_corrupted_contents_stays_in_bounds():
+    """Random words in the descriptor and Elias-Fano arrays: either the stream is
+    rejected, or every host query returns (in bounds of its buffers) some value."""
+    keys = splitmix_keys(22, 3000)
+    data = bytearray(OracleMPHF.build(keys, 8, 100).serialize())
+    rng = np.random.default_rng(0)
+    offs = _vec_offsets(bytes(data))
+    import struct
+    for trial in range(40):
+        bad = bytearray(data)
+        vi = trial % 5
+        n = struct.unpack_from("<Q", bad, offs[vi])[0]
+        for _ in range(4):
+            w = int(rng.integers(0, max(1, n)))
+            struct.pack_into("<Q", bad, offs[vi] + 8 + 8 * w, int(rng.integers(0, 2**63)) * 2 + int(rng.integers(0, 2)))
+        try:
+            rs = gra.GPURecSplit.deserialize(bytes(bad), 8)
+        except gra.RecSplitError as e:
+            assert e.code == 8
+            continue
+        rs.query(keys)  # must not crash
+
+
 def test_bits_count_matches_reference_formula():
     """getBits() = EF bits (DoubleEF.hpp:435-437) + descriptor words * 64."""
     keys = splitmix_keys(4, 10000)

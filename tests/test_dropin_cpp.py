@@ -26,6 +26,28 @@ def test_cpp_load_and_query(tmp_path, leaf, bucket, n):
     assert "load ok" in r.stdout
 
 
+def test_cpp_reads_one_structure_per_extraction(tmp_path):
+    """operator>> consumes exactly one structure: two MPHFs written back to back,
+    then trailing bytes, are read by two extractions and the bytes stay unread."""
+    ka, kb = splitmix_keys(61, 3000), splitmix_keys(62, 777)
+    (tmp_path / "a.bin").write_bytes(ka.astype("<u8").tobytes())
+    (tmp_path / "b.bin").write_bytes(kb.astype("<u8").tobytes())
+    blob = OracleMPHF.build(ka, 8, 100).serialize() + OracleMPHF.build(kb, 8, 100).serialize() + b"TRAILER"
+    (tmp_path / "s.bin").write_bytes(blob)
+    r = _run("load2", tmp_path / "s.bin", tmp_path / "a.bin", tmp_path / "b.bin", 8)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "load2 ok: n=3000,777 trailing=TRAILER" in r.stdout
+
+
+def test_cpp_load_rejects_truncated_stream(tmp_path):
+    keys = splitmix_keys(63, 2000)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    data = OracleMPHF.build(keys, 8, 100).serialize()
+    (tmp_path / "mphf.bin").write_bytes(data[:-5])
+    r = _run("load", tmp_path / "mphf.bin", tmp_path / "keys.bin", 8)
+    assert r.returncode == 3 and "error 8" in r.stdout
+
+
 def test_cpp_load_rejects_leaf_mismatch(tmp_path):
     keys = splitmix_keys(5, 1000)
     (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())

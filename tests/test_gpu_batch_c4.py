@@ -44,3 +44,40 @@ def test_4096_uneven_sets_l12_b100(env):
         k = keys[off[i]:off[i + 1]]
         assert rss[i].serialize() == OracleMPHF.build(k, 12, 100, threads=8).serialize(), f"set {i}"
     sys.stdout.flush()
+
+
+def test_batch_set_above_8192_buckets_uses_global_histogram(env):
+    """A set of 1,000,000 keys at b = 100 has 10,000 buckets, more than the LDS
+    histogram holds: its bucket counts go through global atomics.  Every set's
+    bytes equal the oracle's build of that set alone."""
+    torch, gra, gen = env
+    sizes = [1_000_000, 5000, 300]
+    keys = np.concatenate([gen(500 + i, s).cpu().numpy().view(np.uint64) for i, s in enumerate(sizes)])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    print(f"sets {sizes}", flush=True)
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12)
+    assert st["num_buckets"] == 10000 + 50 + 3
+    for i, rs in enumerate(rss):
+        k = keys[off[i]:off[i + 1]]
+        assert rs.serialize() == OracleMPHF.build(k, 12, 100, threads=8).serialize(), f"set {i}"
+
+
+def test_batch_more_than_4M_buckets_device_scan(env):
+    """Two sets at b = 1 with 4.5M buckets in total: every bucket scan has more
+    than 2048 blocks of 2048 items, so the block sums are scanned in two device
+    levels.  Bytes equal the oracle's."""
+    torch, gra, gen = env
+    sizes = [2_200_000, 2_300_000]
+    keys = np.concatenate([gen(n, n).cpu().numpy().view(np.uint64) for n in sizes])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    print(f"sets {sizes} at b=1", flush=True)
+    rss, st = gra.build_batch(d, off, 1, leaf_size=8)
+    assert st["num_buckets"] == sum(sizes)
+    for i, rs in enumerate(rss):
+        k = keys[off[i]:off[i + 1]]
+        assert rs.serialize() == OracleMPHF.build(k, 8, 1, threads=8).serialize(), f"set {i}"
+    one = gra.GPURecSplit(d[: sizes[0] + sizes[1]], 1, leaf_size=8)  # plain build, 4.5M buckets
+    k = keys[: sizes[0] + sizes[1]]
+    assert one.serialize() == OracleMPHF.build(k, 8, 1, threads=8).serialize()

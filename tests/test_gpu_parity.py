@@ -48,6 +48,9 @@ CASES = [
     (8, 100, 50000), (8, 2, 300), (8, 2000, 20000), (9, 231, 8000), (10, 483, 10000), (11, 75, 5000),
     (12, 100, 30000), (13, 1300, 8000), (14, 231, 4000), (16, 2000, 10000), (16, 100, 5000),
     (18, 50, 3000), (20, 100, 500), (24, 10, 100), (8, 5, 1), (8, 100, 10), (16, 2000, 123),
+    # leaf 2: buckets of 2049 or more keys have more tree nodes than the memo's
+    # 11-bit node field holds
+    (2, 2100, 2100), (2, 2000, 20000), (3, 2900, 2900),
 ]
 
 
@@ -69,14 +72,48 @@ def test_reference_cpu_config_1m_l8_b100(torch_cuda):
     _parity(keys, 8, 100)
 
 
-def test_device_query_matches_host(torch_cuda):
+def test_device_query_matches_oracle(torch_cuda):
+    """k_query on a GPU-built structure returns the oracle's operator() value for
+    every key (AbstractParallelRecSplit.hpp:409-466)."""
     torch = torch_cuda
     keys = splitmix_keys(9, 200_000)
     rs = gra.GPURecSplit(keys, 500, leaf_size=10)
+    ref = OracleMPHF.build(keys, 10, 500, threads=8)
     d = torch.from_numpy(keys.view(np.int64)).cuda()
     out = rs.query_device(d).cpu().numpy().view(np.uint64)
-    assert np.array_equal(out, rs.query(keys))
+    assert np.array_equal(out, ref.query(keys))
     assert np.array_equal(np.sort(out), np.arange(200_000, dtype=np.uint64))
+
+
+@pytest.mark.parametrize("leaf,bucket,n,rotate", [(8, 100, 50_000, True), (16, 2000, 30_000, True),
+                                                  (5, 7, 4000, False), (24, 24, 48, True), (2, 2100, 2100, True)])
+def test_device_query_of_deserialized_oracle_bytes(torch_cuda, leaf, bucket, n, rotate):
+    """A structure loaded from the oracle's bytes (no build on this context) is
+    uploaded on its first device query; results equal the oracle's operator()."""
+    torch = torch_cuda
+    keys = splitmix_keys(300 + leaf, n)
+    ref = OracleMPHF.build(keys, leaf, bucket, rotate=rotate, threads=8)
+    rs = gra.GPURecSplit.deserialize(ref.serialize(), leaf, rotate=rotate)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    out = rs.query_device(d).cpu().numpy().view(np.uint64)
+    assert np.array_equal(out, ref.query(keys))
+    out2 = rs.query_device(d[: n // 2 + 1]).cpu().numpy().view(np.uint64)  # reuses the device copy
+    assert np.array_equal(out2, out[: n // 2 + 1])
+
+
+def test_device_query_of_batch_results(torch_cuda):
+    """Handles returned by a batch build have no device copy until their first
+    device query."""
+    torch = torch_cuda
+    sizes = [5000, 17, 30000]
+    keys = np.concatenate([splitmix_keys(400 + i, s) for i, s in enumerate(sizes)])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    rss, _ = gra.build_batch(d, off, 100, leaf_size=8)
+    for i, rs in enumerate(rss):
+        k = keys[off[i]:off[i + 1]]
+        out = rs.query_device(d[int(off[i]):int(off[i + 1])]).cpu().numpy().view(np.uint64)
+        assert np.array_equal(out, OracleMPHF.build(k, 8, 100).query(k)), f"set {i}"
 
 
 def test_device_resident_keys_build(torch_cuda):
