@@ -21,7 +21,8 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "golomb_memo", "LIB_PATH", "STATUS"]
+__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "build_noalloc", "golomb_memo", "LIB_PATH",
+           "STATUS"]
 
 LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
 
@@ -87,6 +88,8 @@ def _load():
         "rs_mphf_query": (C.c_uint64, [vp, C.c_uint64, C.c_uint64]),
         "rs_mphf_query_batch": (None, [vp, u64p, C.c_size_t, u64p]),
         "rs_query_device": (C.c_int, [vp, vp, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "rs_build_device_noalloc": (C.c_int, [vp, C.c_void_p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
+                                              C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "rs_build_batch": (C.c_int, [vp, C.c_void_p, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
                                      C.POINTER(vp), C.POINTER(BuildStats)]),
         "rs_mphf_bits": (C.c_uint64, [vp]),
@@ -282,3 +285,21 @@ def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: b
            for h in handles]
     return out, st.as_dict()
 
+
+def build_noalloc(d_keys, bucket_size: int, leaf_size: int, rotate: bool = True, ctx: Context | None = None,
+                  out_capacity: int | None = None) -> bytes:
+    """Allocation-free build (rs_build_device_noalloc): the serialized MPHF of the
+    keys in d_keys (a contiguous (n, 2) 64-bit CUDA tensor), built entirely in the
+    library's fixed per-GPU scratch array.  out_capacity bounds the host output
+    buffer (default: 4 bytes per key + 64 KiB, ample for every leaf size)."""
+    ctx = ctx or Context.default()
+    if d_keys.dim() != 2 or d_keys.shape[1] != 2 or d_keys.element_size() != 8 or not d_keys.is_contiguous():
+        raise ValueError("d_keys must be a contiguous (n, 2) 64-bit CUDA tensor")
+    n = d_keys.shape[0]
+    cap = out_capacity if out_capacity is not None else 4 * n + (64 << 10)
+    buf = C.create_string_buffer(cap)
+    ln = C.c_size_t(0)
+    _sync_torch(d_keys)
+    _check(lib.rs_build_device_noalloc(ctx.h, C.c_void_p(d_keys.data_ptr()), n, leaf_size, bucket_size, int(rotate),
+                                       buf, cap, C.byref(ln)))
+    return buf.raw[: ln.value]

@@ -54,12 +54,33 @@ struct BuildError : std::runtime_error {
 // scans, 25 B/bucket + 1 B/key of Elias-Fano arrays, 1 MiB of fixed-size buffers.
 static uint64_t working_set_bound(uint64_t n, uint64_t nb) { return 75 * n + 89 * nb + (uint64_t(1) << 20); }
 
+// Bump allocator over a fixed device region (the allocation-free build): every
+// buffer of the build is carved from it in order; nothing is freed.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t size = 0, used = 0;
+    template <typename T>
+    T *take(size_t count) {
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        const size_t at = (used + 255) & ~size_t(255);
+        if (at + bytes > size)
+            throw BuildError(RS_ERR_ARG, "the build needs more than the " + std::to_string(size >> 20) +
+                                             " MiB scratch of the allocation-free entry point");
+        used = at + bytes;
+        return (T *)(base + at);
+    }
+};
+// Set while the calling thread runs an allocation-free build: DevBuf requests are
+// then served from the arena and the context's own buffers are left untouched.
+static thread_local Arena *t_arena = nullptr;
+
 // Grow-only device allocation reused across builds of a context.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
     template <typename T>
     T *get(size_t count) {
+        if (t_arena) return t_arena->take<T>(count);
         const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
         if (bytes > cap) {
             if (p) HIP_CHECK(hipFree(p));
@@ -238,9 +259,22 @@ static Tune parse_tune(const char *env) {
     return t;
 }
 
+// Scratch of the allocation-free build (rs_build_device_noalloc): a fixed array in
+// the library's code object, one copy per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel: each device's address comes
+// from hipGetSymbolAddress once, at the first context creation on that device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_use[kMaxDevices];  // one allocation-free build per GPU at a time
+
 struct rs_ctx {
     Tune tune;
     int device = 0;
+    uint8_t *scratch = nullptr;  // this device's address of `scratch`
     int num_cus = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -294,6 +328,15 @@ static void scan_u64(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t 
 // Device tables of a leaf size: the Golomb-Rice memo, then the unmasked node
 // count of a bucket by size (kMaxBucket entries each).
 static const uint32_t *ctx_memo(rs_ctx *c, uint32_t leaf) {
+    if (t_arena) {
+        // allocation-free build: the tables are copied into the scratch per call
+        uint32_t *t = t_arena->take<uint32_t>(kMaxBucket * 2);
+        HIP_CHECK(hipMemcpyAsync(t, golomb_memo(leaf).data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIP_CHECK(hipMemcpyAsync(t + kMaxBucket, tree_nodes(leaf).data(), sizeof(uint32_t) * kMaxBucket,
+                                 hipMemcpyHostToDevice, c->stream));
+        return t;
+    }
     if (!c->memo[leaf]) {
         const Memo &m = golomb_memo(leaf);
         const Memo &t = tree_nodes(leaf);
@@ -384,14 +427,25 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
 
 constexpr uint64_t kTileKeys = 1 << 14;  // keys per histogram / scatter workgroup
 
+// Where a single-set build writes its serialized structure instead of returning a
+// handle (the allocation-free entry point): the caller's buffer.
+struct SerialOut {
+    uint8_t *buf;
+    size_t cap;
+    size_t *len;
+};
+
 // Builds one MPHF per key set; set s has set_n[s] keys at set_keys[s] (device).
 // All sets share one leaf size and bucket size; their buckets, tree nodes and
 // seed searches run together.  With device_copy, the (single) result also keeps
-// a device-resident copy for batched GPU queries.
+// a device-resident copy for batched GPU queries.  With so (one set only), the
+// structure is serialized straight from device memory into so->buf and no handle
+// is made.
 static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint64_t *> &set_keys,
                                          const std::vector<uint64_t> &set_n, uint32_t leaf, uint64_t bucket_size,
                                          bool rotate, bool device_copy,
-                                         const std::chrono::steady_clock::time_point t_start) {
+                                         const std::chrono::steady_clock::time_point t_start,
+                                         const SerialOut *so = nullptr) {
     std::vector<uint64_t> set_off(set_n.size() + 1, 0);  // key offsets of the sets in the sorted array
     for (size_t i = 0; i < set_n.size(); i++) set_off[i + 1] = set_off[i] + set_n[i];
     hipStream_t s = c->stream;
@@ -652,10 +706,12 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         ef_words[i][1] = ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys);
         ef_words[i][2] = ef_upper_words(nb, ef.u_position, ef.l_position);
         ef_words[i][3] = enc[i].jump_words;
-        ef.lower_bits.assign(ef_words[i][0], 0);
-        ef.upper_cum.assign(ef_words[i][1], 0);
-        ef.upper_pos.assign(ef_words[i][2], 0);
-        ef.jump.assign(ef_words[i][3], 0);
+        if (!so) {
+            ef.lower_bits.assign(ef_words[i][0], 0);
+            ef.upper_cum.assign(ef_words[i][1], 0);
+            ef.upper_pos.assign(ef_words[i][2], 0);
+            ef.jump.assign(ef_words[i][3], 0);
+        }
         tot_lower += ef_words[i][0];
         tot_uc += ef_words[i][1];
         tot_up += ef_words[i][2];
@@ -675,6 +731,45 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     const int e_enc_end = ev_record(c);
 
     // ---- 8. results to the host
+    if (so) {
+        // serialized layout (AbstractParallelRecSplit.hpp:480-488, DoubleEF.hpp:100-113):
+        // three header words, the descriptor vector, six EF scalars, four EF vectors,
+        // each vector a word count then the words, copied straight from HBM
+        const EliasFano &ef = efs[0];
+        const uint64_t need = 24 + 8 + 8 * rice_total + 48 + 4 * 8 + 8 * (tot_lower + tot_uc + tot_up + tot_jump);
+        *so->len = need;
+        if (so->cap < need)
+            throw BuildError(RS_ERR_ARG, "output buffer of " + std::to_string(so->cap) + " bytes; the structure needs " +
+                                             std::to_string(need));
+        uint8_t *p = so->buf;
+        auto word = [&](uint64_t v) {
+            memcpy(p, &v, 8);
+            p += 8;
+        };
+        auto vec = [&](const void *src, uint64_t words) {
+            word(words);
+            if (words) HIP_CHECK(hipMemcpyAsync(p, src, words * 8, hipMemcpyDeviceToHost, s));
+            p += 8 * words;
+        };
+        word(leaf);
+        word(bucket_size);
+        word(set_off[1] - set_off[0]);
+        vec(rice, rice_total);
+        word(ef.num_buckets);
+        word(ef.u_cum_keys);
+        word(ef.u_position);
+        word((uint64_t)ef.cum_keys_min_delta);
+        word((uint64_t)ef.min_diff);
+        word(ef.bits_per_key_fixed_point);
+        vec(d_lower, tot_lower);
+        vec(d_uc, tot_uc);
+        vec(d_up, tot_up);
+        vec(d_jump, tot_jump);
+        HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (h[16] & 4) throw BuildError(RS_ERR_EF_OVERFLOW, "Elias-Fano jump offset does not fit 16 bits");
+        return {};
+    }
     std::vector<uint64_t> h_rice(rice_total), h_lower(tot_lower), h_uc(tot_uc), h_up(tot_up), h_jump(tot_jump);
     HIP_CHECK(hipMemcpyAsync(h_rice.data(), rice, rice_total * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h_lower.data(), d_lower, tot_lower * 8, hipMemcpyDeviceToHost, s));
@@ -863,6 +958,17 @@ int rs_ctx_create(int device, rs_ctx **out) {
         HIP_CHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
         if (c->wall_khz <= 0) c->wall_khz = 100000;  // s_memrealtime: 100 MHz on gfx9 parts
         if (const char *tl = getenv("RS_TIME_LIMIT")) c->time_limit_s = std::max(0.0, atof(tl));
+        {
+            // the allocation-free entry point's scratch: this device's address, once
+            std::lock_guard<std::mutex> g(g_scratch_mu);
+            if (device >= kMaxDevices) throw HipError(hipErrorInvalidDevice, "device ordinal above 63", __LINE__);
+            if (!g_scratch_ptr[device]) {
+                void *p = nullptr;
+                HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                g_scratch_ptr[device] = (uint8_t *)p;
+            }
+            c->scratch = g_scratch_ptr[device];
+        }
         HIP_CHECK(hipStreamCreate(&c->stream));  // blocking: ordered after null-stream work
         HIP_CHECK(hipHostMalloc((void **)&c->h_pinned, 64 * 1024, hipHostMallocDefault));
         for (auto &e : c->ev) HIP_CHECK(hipEventCreate(&e));
@@ -921,6 +1027,34 @@ int rs_build_device(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_t leaf, 
     const auto t0 = std::chrono::steady_clock::now();
     if (int e = check_args(c, d_keys, n, leaf, bucket, out)) return e;
     return guarded("rs_build_device", c, [&] { *out = build(c, d_keys, n, leaf, bucket, rotate != 0, t0); });
+}
+
+int rs_build_device_noalloc(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_t leaf, size_t bucket, int rotate,
+                            uint8_t *out, size_t cap, size_t *out_len) {
+    const auto t0 = std::chrono::steady_clock::now();
+    rs_mphf *unused = nullptr;
+    if (int e = check_args(c, d_keys, n, leaf, bucket, &unused)) return e;
+    if (!out || !out_len) return set_error("rs_build_device_noalloc: null output"), RS_ERR_ARG;
+    const uint64_t nb = std::max<uint64_t>(1, (n + bucket - 1) / bucket);
+    const uint64_t need = working_set_bound(n, nb);
+    if (need > kNoAllocScratchBytes) {
+        set_error("rs_build_device_noalloc: " + std::to_string(n) + " keys in " + std::to_string(nb) +
+                  " buckets may need " + std::to_string(need) + " bytes of scratch; the maximum is " +
+                  std::to_string(kNoAllocScratchBytes) + " (75 B per key + 89 B per bucket + 1 MiB)");
+        return RS_ERR_ARG;
+    }
+    return guarded("rs_build_device_noalloc", c, [&] {
+        std::lock_guard<std::mutex> g(g_scratch_use[c->device]);
+        Arena a;
+        a.base = c->scratch;
+        a.size = kNoAllocScratchBytes;
+        t_arena = &a;
+        struct Reset {
+            ~Reset() { t_arena = nullptr; }
+        } reset;
+        const SerialOut so{out, cap, out_len};
+        build_sets(c, {d_keys}, {n}, leaf, bucket, rotate != 0, false, t0, &so);
+    });
 }
 
 int rs_build(rs_ctx *c, const uint64_t *keys, size_t n, uint32_t leaf, size_t bucket, int rotate, rs_mphf **out) {

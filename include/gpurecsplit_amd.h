@@ -110,6 +110,22 @@ int rs_build(rs_ctx *ctx, const uint64_t *keys, size_t n, uint32_t leaf_size, si
 int rs_build_device(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf_size,
                     size_t bucket_size, int rotate, rs_mphf **out);
 
+/* Allocation-free construction: the same build as rs_build_device, for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No reference counterpart.  Every device
+ * buffer of the build comes from a fixed 64 MiB scratch array that the library
+ * holds per GPU until the process ends; the call makes no hipMalloc, hipFree or
+ * pinned/torch allocation.  Inputs bounded by 75*n + 89*ceil(n/bucket_size) +
+ * 2^20 <= 64 MiB (about 400,000 keys at any bucket size, 870,000 at b = 100) are
+ * accepted; larger ones fail with RS_ERR_ARG.  The structure is written
+ * serialized (rs_mphf_serialize's layout) into out[0, *out_len); if cap is too
+ * small the call fails with RS_ERR_ARG and *out_len holds the size needed.
+ * All calls on one GPU share its scratch array: the library serialises them with
+ * a lock (overlapping uses of the array would race), and each call clears what it
+ * needs before use. */
+int rs_build_device_noalloc(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf_size, size_t bucket_size,
+                            int rotate, uint8_t *out, size_t cap, size_t *out_len);
+
 /* Batched construction: num_sets independent MPHFs in one set of launches.  Set i
  * owns keys [set_offsets[i], set_offsets[i+1]) of d_keys (device memory;
  * set_offsets is a host array of num_sets+1 increasing offsets, every set
