@@ -21,8 +21,8 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "build_noalloc", "golomb_memo", "LIB_PATH",
-           "STATUS"]
+__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "build_noalloc", "golomb_memo", "hash_strings",
+           "LIB_PATH", "STATUS"]
 
 LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
 
@@ -98,6 +98,13 @@ def _load():
         "rs_mphf_free": (None, [vp]),
         "rs_golomb_memo": (C.c_int, [C.c_uint32, u32p]),
         "rs_last_error": (C.c_char_p, []),
+        "rs_spooky128": (None, [C.c_char_p, C.c_size_t, u64p, u64p]),
+        "rs_first_hash": (None, [C.c_char_p, C.c_size_t, u64p]),
+        "rs_hash_strings": (None, [C.c_char_p, u64p, C.c_size_t, u64p]),
+        "rs_hash_strings_device": (C.c_int, [vp, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "rs_build_strings": (C.c_int, [vp, C.c_char_p, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
+                                       C.POINTER(vp)]),
+        "rs_mphf_query_string": (C.c_uint64, [vp, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -125,6 +132,24 @@ def _sync_torch(t):
     """The library runs on its own stream: finish torch's pending work on t first."""
     import torch
     torch.cuda.current_stream(t.device).synchronize()
+
+
+def _pack_strings(keys) -> tuple[bytes, np.ndarray]:
+    bs = [k.encode() if isinstance(k, str) else bytes(k) for k in keys]
+    off = np.zeros(len(bs) + 1, dtype=np.uint64)
+    if bs:
+        off[1:] = np.cumsum([len(b) for b in bs])
+    return b"".join(bs), off
+
+
+def hash_strings(keys) -> np.ndarray:
+    """The reference's first_hash (SpookyHash V2, seeds 0) of each string or bytes
+    key, on the host: an (n, 2) uint64 array of (first, second)."""
+    blob, off = _pack_strings(keys)
+    out = np.empty((len(off) - 1, 2), dtype=np.uint64)
+    lib.rs_hash_strings(blob, off.ctypes.data_as(C.POINTER(C.c_uint64)), out.shape[0],
+                        out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
 
 
 def golomb_memo(leaf_size: int) -> np.ndarray:
@@ -199,6 +224,23 @@ class GPURecSplit:
             _check(lib.rs_build(self._ctx.h, k.ctypes.data_as(C.POINTER(C.c_uint64)), k.shape[0], leaf_size,
                                 bucket_size, int(rotate), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_strings(cls, keys, bucket_size: int, leaf_size: int = 8, rotate: bool = True,
+                     ctx: Context | None = None) -> "GPURecSplit":
+        """GPURecSplit(const vector<string>&, bucket_size, numThreads) (GPURecSplit.cuh:443-478):
+        string or bytes keys, hashed with SpookyHash V2 on the GPU."""
+        ctx = ctx or Context.default()
+        blob, off = _pack_strings(keys)
+        h = C.c_void_p()
+        _check(lib.rs_build_strings(ctx.h, blob, off.ctypes.data_as(C.POINTER(C.c_uint64)), len(off) - 1, leaf_size,
+                                    bucket_size, int(rotate), C.byref(h)))
+        return cls(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=h)
+
+    def query_string(self, key) -> int:
+        """operator()(const string&) (AbstractParallelRecSplit.hpp:473)."""
+        b = key.encode() if isinstance(key, str) else bytes(key)
+        return int(lib.rs_mphf_query_string(self.h, b, len(b)))
 
     # operator>> (GPURecSplit.cuh:827-841)
     @classmethod

@@ -217,13 +217,16 @@ using namespace rsa;
 // RS_TUNE="u=4,u0=1,lut=1,g1=1,g2=2,g3=4,bpc=8,sk=1" (u: candidates in flight per
 // lane for every kind, uK: for kind K only, lut: LDS increment table for aggregation,
 // sk / skK: keys in scalar registers for every kind / kind K,
-// gK: 64*u-candidate groups per claimed chunk of kind K, bpc: workgroups of 4
-// waves per CU).  Fanout-2 nodes need ~40 candidates, so they run one per lane.
+// gK: 64*u-candidate groups per claimed chunk of kind K, hK: helper cap of kind K
+// (0: none), bpc: workgroups of 4 waves per CU).  Fanout-2 nodes need ~40 candidates, so they run one per lane.
 struct Tune {
-    int u[kNumKinds] = {1, 4, 4, 4, 4};
-    int sk[kNumKinds] = {0, 0, 0, 0, 0};  // keys in scalar registers
+    int u[kNumKinds] = {1, 2, 2, 4, 4};
+    int sk[kNumKinds] = {1, 1, 1, 1, 1};  // keys in scalar registers
     int lut = 1;
     int groups[kNumKinds] = {1, 1, 2, 4, 4};
+    // helpers join a node only while it has fewer waves than this (0: no cap);
+    // fanout-2 nodes need ~40 candidates, so extra waves mostly overshoot
+    int help_cap[kNumKinds] = {2, 0, 0, 0, 0};
     int blocks_per_cu = 8;
 };
 
@@ -253,6 +256,8 @@ static Tune parse_tune(const char *env) {
             else if (k == "bpc" && v >= 1 && v <= 32) t.blocks_per_cu = v;
             else if (k.size() == 2 && k[0] == 'g' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && v >= 1 && v <= 64)
                 t.groups[k[1] - '0'] = v;
+            else if (k.size() == 2 && k[0] == 'h' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && v >= 0)
+                t.help_cap[k[1] - '0'] = v;
         }
         p = q + 1;
     }
@@ -280,7 +285,7 @@ struct rs_ctx {
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
         list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
-        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock;
+        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock, str_bytes, str_off;
     double time_limit_s = 0;  // 0: none
     int wall_khz = 0;         // device wall-clock rate
     uint32_t *memo[kMaxLeaf + 1] = {};
@@ -600,6 +605,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.limit_ticks = limit_ticks;
         a.err = err;
         a.groups = (uint32_t)tn.groups[kind];
+        a.help_cap = (uint32_t)tn.help_cap[kind];
         bool wide = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
@@ -853,7 +859,7 @@ static std::vector<DevBuf *> ctx_bufs(rs_ctx *c) {
             &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
             &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
             &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
-            &c->set_gather, &c->set_enc, &c->bit_off, &c->clock};
+            &c->set_gather, &c->set_enc, &c->bit_off, &c->clock, &c->str_bytes, &c->str_off};
 }
 
 static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn) {
@@ -1054,6 +1060,38 @@ int rs_build_device_noalloc(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_
         } reset;
         const SerialOut so{out, cap, out_len};
         build_sets(c, {d_keys}, {n}, leaf, bucket, rotate != 0, false, t0, &so);
+    });
+}
+
+int rs_hash_strings_device(rs_ctx *c, const uint8_t *d_bytes, const uint64_t *d_offsets, size_t n, uint64_t *d_keys) {
+    if (!c || (n && (!d_bytes || !d_offsets || !d_keys))) return set_error("rs_hash_strings_device: null argument"), RS_ERR_ARG;
+    if (n == 0) return RS_OK;
+    return guarded("rs_hash_strings_device", c, [&] {
+        k_spooky_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(d_bytes, d_offsets, n, d_keys);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rs_build_strings(rs_ctx *c, const uint8_t *bytes, const uint64_t *offsets, size_t n, uint32_t leaf, size_t bucket,
+                     int rotate, rs_mphf **out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!offsets || (!bytes && n && offsets[n] > offsets[0])) return set_error("rs_build_strings: null argument"), RS_ERR_ARG;
+    if (int e = check_args(c, offsets, n, leaf, bucket, out)) return e;
+    for (size_t i = 0; i < n; i++)
+        if (offsets[i + 1] < offsets[i]) return set_error("rs_build_strings: decreasing offsets"), RS_ERR_ARG;
+    return guarded("rs_build_strings", c, [&] {
+        const uint64_t base = offsets[0], total = offsets[n] - base;
+        uint8_t *d_bytes = c->str_bytes.get<uint8_t>(total + 8);
+        uint64_t *d_off = c->str_off.get<uint64_t>(n + 1);
+        uint64_t *d_keys = c->keys_in.get<uint64_t>(2 * n);
+        std::vector<uint64_t> rel(offsets, offsets + n + 1);
+        for (uint64_t &o : rel) o -= base;
+        if (total) HIP_CHECK(hipMemcpyAsync(d_bytes, bytes + base, total, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(d_off, rel.data(), (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
+        k_spooky_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(d_bytes, d_off, n, d_keys);
+        HIP_CHECK(hipGetLastError());
+        *out = build(c, d_keys, n, leaf, bucket, rotate != 0, t0);
     });
 }
 

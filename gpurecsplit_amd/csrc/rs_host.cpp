@@ -5,6 +5,7 @@
 #include <mutex>
 
 #include "rs_internal.h"
+#include "rs_spooky.h"
 
 namespace rsa {
 
@@ -452,6 +453,25 @@ int rs_mphf_deserialize(const uint8_t *buf, size_t len, uint32_t leaf_size, int 
 }
 
 uint64_t rs_mphf_query(const rs_mphf *h, uint64_t first, uint64_t second) { return query(*h, first, second); }
+
+void rs_spooky128(const void *data, size_t len, uint64_t *hash1, uint64_t *hash2) {
+    spooky::hash128((const uint8_t *)data, len, *hash1, *hash2);
+}
+
+void rs_first_hash(const void *data, size_t len, uint64_t *out2) {
+    spooky::first_hash((const uint8_t *)data, len, out2[0], out2[1]);
+}
+
+void rs_hash_strings(const uint8_t *bytes, const uint64_t *offsets, size_t n, uint64_t *out) {
+    for (size_t i = 0; i < n; i++)
+        spooky::first_hash(bytes + offsets[i], offsets[i + 1] - offsets[i], out[2 * i], out[2 * i + 1]);
+}
+
+uint64_t rs_mphf_query_string(const rs_mphf *h, const void *data, size_t len) {
+    uint64_t first, second;
+    spooky::first_hash((const uint8_t *)data, len, first, second);
+    return query(*h, first, second);
+}
 
 void rs_mphf_query_batch(const rs_mphf *h, const uint64_t *keys, size_t n, uint64_t *out) {
     for (size_t i = 0; i < n; i++) out[i] = query(*h, keys[2 * i], keys[2 * i + 1]);

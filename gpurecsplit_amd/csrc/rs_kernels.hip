@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "rs_common.h"
+#include "rs_spooky.h"
 
 namespace rsa {
 
@@ -69,6 +70,12 @@ __device__ __forceinline__ uint32_t hash_pre32(uint32_t t) {
     return t * 0xc2b2ae35u;
 }
 __device__ __forceinline__ uint32_t hash_pre(uint64_t z) { return hash_pre32((uint32_t)(z >> 32) ^ (uint32_t)z); }
+// hash_pre32 after its first step: t already holds t ^ (t >> 16).
+__device__ __forceinline__ uint32_t hash_post16(uint32_t t) {
+    t *= 0x85ebca6bu;
+    t ^= t >> 13;
+    return t * 0xc2b2ae35u;
+}
 __device__ __forceinline__ uint32_t h16_of(uint32_t t) {
     uint32_t x;
     __asm__("v_xor_b32_sdwa %0, %1, %1 dst_sel:WORD_0 dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:WORD_0"
@@ -293,6 +300,7 @@ struct SearchArgs {
     uint32_t magic;       // ceil(2^20 / split_unit)
     uint32_t upper;       // upper aggregation bound (fanout-2 split size)
     uint32_t lut_stride;  // LDS words per wave for the aggregation increment table
+    uint32_t help_cap;    // helpers join only nodes with fewer waves than this (0: no cap)
     // Time limit: 0 = none; otherwise waves leave the search once the device wall
     // clock passes *clock0 + limit_ticks, setting bit 3 of *err.
     const unsigned long long *clock0;
@@ -395,11 +403,14 @@ struct Seeds {
 
 // Calls g(u, h16) for every key of kp[begin, end) and every candidate u, where
 // h16 = fold_h16(key + base + off[u]) (rs_common.h).  SK: keys in scalar
-// registers; key + base is then one scalar 64-bit add, and while its low word
-// plus max_off does not carry, key + base + off[u] has that sum as low word and
-// the uniform high word, so each candidate costs one 32-bit add instead of a
-// 64-bit one.  Otherwise (rare: within max_off of a 2^32 boundary) the full
-// 64-bit add.  Without SK: vector key loads and per-lane 64-bit seeds.
+// registers; S = key + base is one scalar 64-bit add.  With lo, hi the halves of
+// S: while (lo mod 2^16) + max_off < 2^16, adding off[u] changes neither hi nor
+// lo >> 16, so the first two steps of the fold, t = (lo+off) ^ hi and
+// t ^ (t >> 16), reduce to (lo + off) ^ D with the wave-uniform
+// D = hi ^ (hi >> 16) ^ (lo >> 16): one 32-bit add and one xor per candidate.
+// Past a 2^16 boundary but not a 2^32 one, t is (lo + off) ^ hi; otherwise (within
+// max_off of a 2^32 boundary) the full 64-bit add.  Without SK: vector key
+// loads and per-lane 64-bit seeds.
 template <int U, bool SK, typename G>
 __device__ __forceinline__ void each_hash(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end,
                                           const Seeds<U> &sd, G &&g) {
@@ -409,7 +420,11 @@ __device__ __forceinline__ void each_hash(const uint64_t *__restrict__ kp, uint3
         stream_keys_scalar(kp, begin, end, [&](uint64_t k) {
             const uint64_t S = k + base;
             const uint32_t lo = (uint32_t)S, hi = (uint32_t)(S >> 32);
-            if (lo <= ~max_off) {
+            if ((lo & 0xFFFFu) <= 0xFFFFu - max_off) {
+                const uint32_t D = hi ^ (hi >> 16) ^ (lo >> 16);
+#pragma unroll
+                for (int u = 0; u < U; u++) g(u, h16_of(hash_post16((lo + sd.off[u]) ^ D)));
+            } else if (lo <= ~max_off) {
 #pragma unroll
                 for (int u = 0; u < U; u++) g(u, h16_of(hash_pre32((lo + sd.off[u]) ^ hi)));
             } else {
@@ -490,25 +505,21 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
     const Seeds<U> sd(u0 * L + S, L);
     each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
     each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
+    // Candidates whose two masks are collision-free are tested in index order, one
+    // at a time across the wave: lane r tests rotation r, so one step covers all L
+    // rotations and the lowest fitting rotation is the lowest set bit of a ballot.
+    const uint32_t r = lane_id() < L ? lane_id() : 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
-        if (!__ballot(ok)) continue;
-        uint32_t rf = 0xFFFFFFFFu;
-        if (ok) {
-            for (uint32_t r = 0; r < L; r++) {
-                const uint32_t rot = r == 0 ? mr[u] : (((mr[u] << r) | (mr[u] >> (L - r))) & full);
-                if ((ml[u] | rot) == full) {
-                    rf = r;
-                    break;
-                }
-            }
-        }
-        const unsigned long long msk = __ballot(rf != 0xFFFFFFFFu);
-        if (msk) {
-            const int l = __ffsll((long long)msk) - 1;
-            const uint32_t r = uni(__shfl(rf, l));
-            return (u0 + (uint64_t)(u * 64 + l)) * L + r;
+        unsigned long long cand = __ballot((uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr);
+        while (cand) {
+            const int l = __ffsll((long long)cand) - 1;
+            cand &= cand - 1;
+            const uint32_t bl = (uint32_t)__builtin_amdgcn_readlane((int)ml[u], l);
+            const uint32_t br = (uint32_t)__builtin_amdgcn_readlane((int)mr[u], l);
+            const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
+            const unsigned long long fit = __ballot(lane_id() < L && (bl | rot) == full);
+            if (fit) return (u0 + (uint64_t)(u * 64 + l)) * L + (uint64_t)(__ffsll((long long)fit) - 1);
         }
     }
     return kNone;
@@ -588,14 +599,16 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                     while (idx >= a.n_waves) idx -= a.n_waves;
                     int cand = -1;
                     if (k + lane < a.n_waves) cand = __hip_atomic_load(&a.running[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const bool ok = cand >= 0 && has_work(&a.state[cand], units_per_chunk, scale);
+                    bool ok = cand >= 0 && has_work(&a.state[cand], units_per_chunk, scale);
+                    uint32_t w = 0;
+                    if (ok) {
+                        w = __hip_atomic_load(&a.state[cand].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = a.help_cap == 0 || w < a.help_cap;
+                    }
                     const unsigned long long msk = __ballot(ok);
                     if (msk) {
                         uint32_t key = 0xFFFFFFFFu;
-                        if (ok) {
-                            const uint32_t w = __hip_atomic_load(&a.state[cand].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            key = (min(w, 0xFFFFu) << 16) | lane;
-                        }
+                        if (ok) key = (min(w, 0xFFFFu) << 16) | lane;
 #pragma unroll
                         for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o));
                         const int l = (int)(uni(key) & 0xFFFF);
@@ -1037,6 +1050,23 @@ __global__ __launch_bounds__(256) void k_ef_fill(const SetEnc *__restrict__ sets
             }
         }
     }
+}
+
+// ------------------------------------------------------------------ string keys
+
+// One thread per string key: the reference's first_hash (SpookyHash V2 128-bit,
+// seeds 0; AbstractParallelRecSplit.hpp:250, include/util/Hash128.h:32-36).  Key i
+// is bytes [offsets[i], offsets[i+1]) of `bytes`; out receives (first, second).
+__global__ __launch_bounds__(256) void k_spooky_keys(const uint8_t *__restrict__ bytes,
+                                                     const uint64_t *__restrict__ offsets, uint64_t n,
+                                                     uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t b = offsets[i];
+    uint64_t first, second;
+    spooky::first_hash(bytes + b, (size_t)(offsets[i + 1] - b), first, second);
+    out[2 * i] = first;
+    out[2 * i + 1] = second;
 }
 
 // ------------------------------------------------------------------ queries

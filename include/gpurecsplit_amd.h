@@ -216,6 +216,30 @@ int rs_poll(rs_server *s, uint64_t job_id, rs_job_info *info);
 int rs_wait(rs_server *s, uint64_t job_id, double timeout_s, rs_mphf **out, rs_job_info *info);
 int rs_server_stats(rs_server *s, rs_serve_stats *out);
 
+/* ---------------------------------------------------------------------------
+ * String keys.  The reference replaces every string key by the 128-bit hash
+ * first_hash(key, len) = SpookyHash V2 Hash128 with seeds 0, taken as
+ * {first = h1, second = h0} (AbstractParallelRecSplit.hpp:250,
+ * include/util/Hash128.h:32-36); these entry points do the same.
+ * ------------------------------------------------------------------------- */
+
+/* SpookyHash::Hash128: *hash1, *hash2 are the seeds on entry and the hash on exit. */
+void rs_spooky128(const void *data, size_t len, uint64_t *hash1, uint64_t *hash2);
+/* first_hash of one key: out2[0] = first, out2[1] = second. */
+void rs_first_hash(const void *data, size_t len, uint64_t *out2);
+/* first_hash of n keys on the host: key i is bytes [offsets[i], offsets[i+1]). */
+void rs_hash_strings(const uint8_t *bytes, const uint64_t *offsets, size_t n, uint64_t *out);
+/* The same on the GPU, everything in device memory of ctx's GPU. */
+int rs_hash_strings_device(rs_ctx *ctx, const uint8_t *d_bytes, const uint64_t *d_offsets, size_t n,
+                           uint64_t *d_keys);
+/* Construction from string keys (host bytes + n+1 offsets): the keys are hashed on
+ * the GPU, then built as rs_build does.  Replaces GPURecSplit(const vector<string>&,
+ * size_t bucket_size, int numThreads), include/function/GPURecSplit.cuh:443-478. */
+int rs_build_strings(rs_ctx *ctx, const uint8_t *bytes, const uint64_t *offsets, size_t n, uint32_t leaf_size,
+                     size_t bucket_size, int rotate, rs_mphf **out);
+/* operator()(const string&), AbstractParallelRecSplit.hpp:473. */
+uint64_t rs_mphf_query_string(const rs_mphf *h, const void *data, size_t len);
+
 /* Serialization in the reference's stream layout. */
 size_t rs_mphf_serialized_size(const rs_mphf *h);
 int rs_mphf_serialize(const rs_mphf *h, uint8_t *buf, size_t cap);

@@ -16,7 +16,8 @@
 //   - The caller's key vector is not reordered (the reference sorts it in place).
 //   - Failures throw gpurecsplit_amd::Error instead of exit()/abort() or looping
 //     forever on duplicate keys.
-//   - The string-key constructors (SpookyHash V2 front end) are not provided.
+//   - String keys are hashed with the library's SpookyHash V2 (rs_first_hash), on the
+//     GPU for construction; the result is the reference's first_hash.
 //
 // Link with -lrecsplit_amd (gpurecsplit_amd/librecsplit_amd.so).  The GPU is the
 // one selected by the environment variable RS_DEVICE (default 0).
@@ -26,6 +27,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <istream>
 #include <iterator>
 #include <mutex>
@@ -106,6 +108,21 @@ class GPURecSplit {
                                "GPURecSplit");
     }
 
+    /** Builds the MPHF of distinct string keys (GPURecSplit.cuh:443-478): every key is
+     *  replaced by its 128-bit SpookyHash V2 first_hash, computed on the GPU. */
+    GPURecSplit(const std::vector<std::string> &keys, const size_t bucket_size, const int numThreads = 1) {
+        (void)numThreads;
+        build_strings(keys, bucket_size);
+    }
+
+    /** Builds from the keys of a stream, one per line (GPURecSplit.cuh:498-505). */
+    GPURecSplit(std::ifstream &input, const size_t bucket_size, const int numThreads = 1) {
+        (void)numThreads;
+        std::vector<std::string> keys;
+        for (std::string key; std::getline(input, key);) keys.push_back(std::move(key));
+        build_strings(keys, bucket_size);
+    }
+
     /** Builds from keys already resident in device memory of the context's GPU. */
     static GPURecSplit fromDevice(const hash128_t *d_keys, size_t n, size_t bucket_size) {
         GPURecSplit r;
@@ -132,6 +149,9 @@ class GPURecSplit {
     /** Returns the value in [0, n) associated with the given 128-bit hash. */
     size_t operator()(const hash128_t &hash) { return (size_t)rs_mphf_query(h, hash.first, hash.second); }
 
+    /** Returns the value in [0, n) associated with a string key (AbstractParallelRecSplit.hpp:473). */
+    size_t operator()(const std::string &key) { return (size_t)rs_mphf_query_string(h, key.data(), key.size()); }
+
     /** Bytes of the reference's AbstractParallelRecSplit object, the sizeof term of its
      *  getBits(): bucket_size, nbuckets, keys_count (3 words), the RiceBitVector (one
      *  sux::util::Vector) and the DoubleEF (4 Vectors and 10 words), with a
@@ -152,6 +172,20 @@ class GPURecSplit {
     }
 
     const rs_mphf *handle() const { return h; }
+
+  private:
+    void build_strings(const std::vector<std::string> &keys, const size_t bucket_size) {
+        std::vector<uint64_t> offsets(keys.size() + 1, 0);
+        for (size_t i = 0; i < keys.size(); i++) offsets[i + 1] = offsets[i] + keys[i].size();
+        std::vector<uint8_t> bytes(offsets.back());
+        for (size_t i = 0; i < keys.size(); i++)
+            if (!keys[i].empty()) std::memcpy(bytes.data() + offsets[i], keys[i].data(), keys[i].size());
+        gpurecsplit_amd::check(rs_build_strings(gpurecsplit_amd::default_context(), bytes.data(), offsets.data(),
+                                                keys.size(), LEAF_SIZE, bucket_size, USE_BIJECTIONS_ROTATE ? 1 : 0, &h),
+                               "GPURecSplit(strings)");
+    }
+
+  public:
 
     friend std::ostream &operator<<(std::ostream &os, const GPURecSplit &rs) {
         std::vector<uint8_t> buf(rs_mphf_serialized_size(rs.h));

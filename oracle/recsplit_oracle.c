@@ -28,6 +28,11 @@
  *                                       reference function than the builder)
  *   - Elias-Fano ...................... get(i) round trip on every bucket
  *   - thread-count invariance ......... byte equality across thread counts
+ *   - 64-bit remap (RecSplitRotate) ... Stafford finalizer pinned by the
+ *                                       SplitMix64 outputs (same function);
+ *                                       the 48-bit scaling against 128-bit
+ *                                       arithmetic; searches by brute-force
+ *                                       minimality; bijection via the query
  *   - serialized framing of sux::util::Vector (u64 length + raw words) and
  *     its resize() semantics: PARITY UNPINNED (sux is absent from the
  *     reference checkout; no reference test or fixture pins it).
@@ -48,6 +53,12 @@
 #define ORC_MAX_LEAF_SIZE 24     /* AbstractParallelRecSplit.hpp:62 */
 #define ORC_MAX_FANOUT 9         /* AbstractParallelRecSplit.hpp:63 */
 #define LANES 16
+
+/* `rotate` arguments and fields are a flags word: bit 0 rotation fitting for full
+ * leaves (USE_BIJECTIONS_ROTATE), bit 1 the 64-bit remap of the RecSplitRotate
+ * variant (USE_64_BIT_REMAP). */
+#define ORC_FLAG_ROTATE 1
+#define ORC_FLAG_REMAP64 2
 
 /* AbstractParallelRecSplit.hpp:95-98 */
 static const uint64_t start_seed[20] = {
@@ -90,6 +101,15 @@ static inline uint32_t remap32(uint64_t x, uint32_t n) {
     return (h * n) >> 16;
 }
 uint32_t orc_remap(uint64_t x, uint32_t n) { return remap32(x, n); }
+
+/* remixAndRemap with USE_64_BIT_REMAP=true (the RecSplitRotate variant,
+ * RecSplitRotate.hpp:115 -> AbstractParallelRecSplit.hpp:327-332): the 64-bit
+ * Stafford finalizer, its low 48 bits scaled to [0, n). */
+static inline uint32_t remap64s(uint64_t x, uint32_t n) {
+    return (uint32_t)(((orc_remix(x) & ((UINT64_C(1) << 48) - 1)) * n) >> 48);
+}
+uint32_t orc_remap64(uint64_t x, uint32_t n) { return remap64s(x, n); }
+static inline uint32_t remap_any(uint64_t x, uint32_t n, int r64) { return r64 ? remap64s(x, n) : remap32(x, n); }
 
 /* common.hpp:273-280 */
 static inline uint64_t remap128(uint64_t x, uint64_t n) {
@@ -223,13 +243,21 @@ static inline u32x8 h16x8(u64x8 z) {
     t ^= t >> 16;
     return t & 0xFFFFu;
 }
-static inline u32x8 remap_x8(u64x8 z, uint32_t n) { return (h16x8(z) * n) >> 16; }
+static inline u32x8 remap16_x8(u64x8 z, uint32_t n) { return (h16x8(z) * n) >> 16; }
+static inline u32x8 remap64_x8(u64x8 z, uint32_t n) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return __builtin_convertvector(((z & ((UINT64_C(1) << 48) - 1)) * n) >> 48, u32x8);
+}
+/* the search loops below are specialised for r64 = 0 (16-bit remap) and r64 = 1 */
+#define remap_x8(z, n) (r64 ? remap64_x8((z), (n)) : remap16_x8((z), (n)))
 static inline u64x8 bcast64(uint64_t v) { return (u64x8){v, v, v, v, v, v, v, v}; }
 
 /* higherLevel, SIMDRecSplit.hpp:250-317: lowest x >= 0 with
  * #{ remap(key + start_seed[level] + x, m) < split } == split. */
-uint64_t orc_search_higher(const uint64_t *keys, uint32_t m, uint32_t split, int level,
-                           uint64_t *evals) {
+static inline uint64_t search_higher(const uint64_t *keys, uint32_t m, uint32_t split, int level,
+                                     uint64_t *evals, const int r64) {
     const uint64_t base = start_seed[level];
     const u64x8 off1 = LANE_OFF0 + 8;
     for (uint64_t x = 0;; x += LANES) {
@@ -291,8 +319,8 @@ static inline uint32_t split_magic(uint32_t split) { return ((1u << 20) + split 
 static const u32x8 INC_LO_A = {1u, 1u << 8, 1u << 16, 1u << 24, 0, 0, 0, 0}, INC_LO_B = {0};
 static const u32x8 INC_HI_A = {0, 0, 0, 0, 1u, 1u << 8, 1u << 16, 1u << 24}, INC_HI_B = {0};
 
-uint64_t orc_search_aggr(const uint64_t *keys, uint32_t m, uint32_t split, uint64_t seed,
-                         uint64_t *evals) {
+static inline uint64_t search_aggr(const uint64_t *keys, uint32_t m, uint32_t split, uint64_t seed,
+                                   uint64_t *evals, const int r64) {
     const uint32_t fanout = (uint16_t)(m + split - 1) / split;
     uint32_t tlo, thi;
     aggr_targets(m, split, fanout, &tlo, &thi);
@@ -347,7 +375,7 @@ int orc_packed_aggr_predicate(const uint32_t *parts, uint32_t m, uint32_t split)
  * keys with even `second` form the left set, odd the right set.  Bases step by
  * L from start_seed[19]; at a base, the lowest rotation r in [0, L) with
  * mask_left | rotl(mask_right, r) == 2^L-1 wins; the stored seed is base + r. */
-uint64_t orc_search_leaf_rot(const uint64_t *keys, uint32_t L, uint64_t *evals) {
+static inline uint64_t search_leaf_rot(const uint64_t *keys, uint32_t L, uint64_t *evals, const int r64) {
     const uint64_t SEED = start_seed[NUM_START_SEEDS - 1];
     const uint32_t full = (uint32_t)((1ULL << L) - 1);
     uint64_t left[ORC_MAX_LEAF_SIZE], right[ORC_MAX_LEAF_SIZE];
@@ -396,7 +424,7 @@ static uint32_t bij_midstop(uint32_t s) {
  * OR_i 1 << remap(key_i + start_seed[19] + x, m) == 2^m - 1.  The midstop +
  * backlog of the reference is a lossless filter; this restatement applies the
  * midstop per lane and then finishes the survivors. */
-uint64_t orc_search_leaf_brute(const uint64_t *keys, uint32_t m, uint64_t *evals) {
+static inline uint64_t search_leaf_brute(const uint64_t *keys, uint32_t m, uint64_t *evals, const int r64) {
     const uint64_t SEED = start_seed[NUM_START_SEEDS - 1];
     const uint32_t found = (uint32_t)((1ULL << m) - 1);
     const uint32_t midstop = m >= 12 ? bij_midstop(m) : m;
@@ -411,13 +439,40 @@ uint64_t orc_search_leaf_brute(const uint64_t *keys, uint32_t m, uint64_t *evals
         for (int j = 0; j < LANES; j++) {
             uint32_t mk = j < 8 ? mk0[j] : mk1[j - 8];
             if ((uint32_t)__builtin_popcount(mk) != midstop) continue;
-            for (uint32_t i = midstop; i < m; i++) mk |= 1u << remap32(keys[i] + SEED + x + j, m);
+            for (uint32_t i = midstop; i < m; i++) mk |= 1u << remap_any(keys[i] + SEED + x + j, m, r64);
             if (mk == found) {
                 if (evals) *evals += (x + j + 1) * (uint64_t)m;
                 return x + j;
             }
         }
     }
+}
+
+/* Exported searches: flags bit 1 selects the 64-bit remap (r64); each pair of
+ * calls below is specialised at compile time. */
+uint64_t orc_search_higher(const uint64_t *keys, uint32_t m, uint32_t split, int level, uint64_t *evals) {
+    return search_higher(keys, m, split, level, evals, 0);
+}
+uint64_t orc_search_aggr(const uint64_t *keys, uint32_t m, uint32_t split, uint64_t seed, uint64_t *evals) {
+    return search_aggr(keys, m, split, seed, evals, 0);
+}
+uint64_t orc_search_leaf_rot(const uint64_t *keys, uint32_t L, uint64_t *evals) {
+    return search_leaf_rot(keys, L, evals, 0);
+}
+uint64_t orc_search_leaf_brute(const uint64_t *keys, uint32_t m, uint64_t *evals) {
+    return search_leaf_brute(keys, m, evals, 0);
+}
+uint64_t orc_search_higher_r64(const uint64_t *keys, uint32_t m, uint32_t split, int level, uint64_t *evals) {
+    return search_higher(keys, m, split, level, evals, 1);
+}
+uint64_t orc_search_aggr_r64(const uint64_t *keys, uint32_t m, uint32_t split, uint64_t seed, uint64_t *evals) {
+    return search_aggr(keys, m, split, seed, evals, 1);
+}
+uint64_t orc_search_leaf_rot_r64(const uint64_t *keys, uint32_t L, uint64_t *evals) {
+    return search_leaf_rot(keys, L, evals, 1);
+}
+uint64_t orc_search_leaf_brute_r64(const uint64_t *keys, uint32_t m, uint64_t *evals) {
+    return search_leaf_brute(keys, m, evals, 1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -500,8 +555,10 @@ static void push_seed(bucket_ctx *c, uint64_t x, uint32_t m) {
 
 static void leaf_level(bucket_ctx *c, uint64_t *bucket, uint32_t m) {
     uint64_t x;
-    if (c->rotate && m == (uint32_t)c->P->leaf) x = orc_search_leaf_rot(bucket, m, &c->evals);
-    else x = orc_search_leaf_brute(bucket, m, &c->evals);
+    const int r64 = (c->rotate & ORC_FLAG_REMAP64) != 0;
+    if ((c->rotate & ORC_FLAG_ROTATE) && m == (uint32_t)c->P->leaf)
+        x = r64 ? orc_search_leaf_rot_r64(bucket, m, &c->evals) : orc_search_leaf_rot(bucket, m, &c->evals);
+    else x = r64 ? orc_search_leaf_brute_r64(bucket, m, &c->evals) : orc_search_leaf_brute(bucket, m, &c->evals);
     push_seed(c, x, m);
 }
 
@@ -514,7 +571,7 @@ static void partition(bucket_ctx *c, uint64_t *bucket, uint32_t m, uint64_t seed
     uint32_t pos[ORC_MAX_FANOUT + 1];
     for (uint32_t p = 0; p < fan; p++) pos[p] = is_higher ? (p ? split_or_unit : 0) : p * split_or_unit;
     for (uint32_t i = 0; i < m; i++) {
-        const uint32_t r = remap32(bucket[i] + seed_abs, m);
+        const uint32_t r = remap_any(bucket[i] + seed_abs, m, (c->rotate & ORC_FLAG_REMAP64) != 0);
         const uint32_t p = is_higher ? (r >= split_or_unit) : r / split_or_unit;
         temp[pos[p]++] = bucket[i];
     }
@@ -523,20 +580,23 @@ static void partition(bucket_ctx *c, uint64_t *bucket, uint32_t m, uint64_t seed
 
 static void rec_split(bucket_ctx *c, uint64_t *bucket, uint32_t m, int level) {
     const orc_params *P = c->P;
+    const int r64 = (c->rotate & ORC_FLAG_REMAP64) != 0;
     if (m <= (uint32_t)P->leaf) {
         leaf_level(c, bucket, m);
         return;
     }
     if (m > P->upper_aggr) { /* fanout 2: SIMDRecSplit.hpp:631-639 */
         const uint32_t split = higher_split(P, m);
-        const uint64_t x = orc_search_higher(bucket, m, split, level, &c->evals);
+        const uint64_t x = r64 ? orc_search_higher_r64(bucket, m, split, level, &c->evals)
+                               : orc_search_higher(bucket, m, split, level, &c->evals);
         push_seed(c, x, m);
         partition(c, bucket, m, start_seed[level] + x, split, 1);
         rec_split(c, bucket, split, level + 1);
         if (m - split > 1) rec_split(c, bucket + split, m - split, level + 1);
     } else if (m > P->lower_aggr) { /* SIMDRecSplit.hpp:640-650 */
         const uint64_t S = start_seed[NUM_START_SEEDS - 3];
-        const uint64_t x = orc_search_aggr(bucket, m, P->lower_aggr, S, &c->evals);
+        const uint64_t x = r64 ? orc_search_aggr_r64(bucket, m, P->lower_aggr, S, &c->evals)
+                               : orc_search_aggr(bucket, m, P->lower_aggr, S, &c->evals);
         push_seed(c, x, m);
         partition(c, bucket, m, S + x, P->lower_aggr, 0);
         uint32_t i;
@@ -546,7 +606,7 @@ static void rec_split(bucket_ctx *c, uint64_t *bucket, uint32_t m, int level) {
     } else { /* SIMDRecSplit.hpp:651-661 */
         const uint64_t S = start_seed[NUM_START_SEEDS - 2];
         const uint32_t L = (uint32_t)P->leaf;
-        const uint64_t x = orc_search_aggr(bucket, m, L, S, &c->evals);
+        const uint64_t x = r64 ? orc_search_aggr_r64(bucket, m, L, S, &c->evals) : orc_search_aggr(bucket, m, L, S, &c->evals);
         push_seed(c, x, m);
         partition(c, bucket, m, S + x, L, 0);
         uint32_t i;
@@ -1043,6 +1103,7 @@ static void rr_skip(rreader *r, uint64_t nodes, uint64_t fixed_len) {
 
 uint64_t orc_query(const orc_mphf *R, uint64_t first, uint64_t second) {
     const orc_params *P = &R->P;
+    const int r64 = (R->rotate & ORC_FLAG_REMAP64) != 0;
     const uint64_t bucket = remap128(first, R->nbuckets);
     uint64_t cum_keys, cum_keys_next, bit_pos;
     ef_get(&R->ef, bucket, &cum_keys, &cum_keys_next, &bit_pos);
@@ -1052,7 +1113,7 @@ uint64_t orc_query(const orc_mphf *R, uint64_t first, uint64_t second) {
     int level = 0;
     while (m > P->upper_aggr) {
         const uint64_t d = rr_next(&rd, (int)golomb_param(P, m));
-        const uint32_t hmod = remap32(second + d + start_seed[level], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + start_seed[level], (uint32_t)m, r64);
         const uint32_t split = ((uint16_t)(m / 2 + P->upper_aggr - 1) / P->upper_aggr) * P->upper_aggr;
         if (hmod < split) {
             m = split;
@@ -1065,7 +1126,7 @@ uint64_t orc_query(const orc_mphf *R, uint64_t first, uint64_t second) {
     }
     if (m > P->lower_aggr) {
         const uint64_t d = rr_next(&rd, (int)golomb_param(P, m));
-        const uint32_t hmod = remap32(second + d + start_seed[NUM_START_SEEDS - 3], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + start_seed[NUM_START_SEEDS - 3], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / P->lower_aggr;
         m = P->lower_aggr < m - part * P->lower_aggr ? P->lower_aggr : m - part * P->lower_aggr;
         cum_keys += P->lower_aggr * part;
@@ -1073,7 +1134,7 @@ uint64_t orc_query(const orc_mphf *R, uint64_t first, uint64_t second) {
     }
     if (m > (uint64_t)P->leaf) {
         const uint64_t d = rr_next(&rd, (int)golomb_param(P, m));
-        const uint32_t hmod = remap32(second + d + start_seed[NUM_START_SEEDS - 2], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + start_seed[NUM_START_SEEDS - 2], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / (uint64_t)P->leaf;
         m = (uint64_t)P->leaf < m - part * P->leaf ? (uint64_t)P->leaf : m - part * P->leaf;
         cum_keys += (uint64_t)P->leaf * part;
@@ -1082,12 +1143,12 @@ uint64_t orc_query(const orc_mphf *R, uint64_t first, uint64_t second) {
     const uint64_t b = rr_next(&rd, (int)golomb_param(P, m));
     const uint64_t x = second + b + start_seed[NUM_START_SEEDS - 1];
     uint64_t leaf_val;
-    if (R->rotate && m == (uint64_t)P->leaf) {
+    if ((R->rotate & ORC_FLAG_ROTATE) && m == (uint64_t)P->leaf) {
         const uint64_t rotation = b % (uint64_t)P->leaf;
-        leaf_val = remap32(x - rotation, (uint32_t)P->leaf);
+        leaf_val = remap_any(x - rotation, (uint32_t)P->leaf, r64);
         if (second % 2 != 0) leaf_val = (leaf_val + rotation) % (uint64_t)P->leaf;
     } else {
-        leaf_val = remap32(x, (uint32_t)m);
+        leaf_val = remap_any(x, (uint32_t)m, r64);
     }
     return cum_keys + leaf_val;
 }

@@ -4,6 +4,9 @@
 //   dropin_test load2 <stream.bin> <keysA.bin> <keysB.bin> <leaf>
 //                                                   two structures back to back, then trailing
 //                                                   bytes: operator>> reads exactly one each (no GPU)
+//   dropin_test loadstr <mphf.bin> <keys.txt> <leaf>   operator>> + operator()(string) (no GPU)
+//   dropin_test buildstr <keys.txt> <leaf> <bucket> <out.bin>
+//                                                   string constructor from an ifstream (GPU)
 //   dropin_test build <keys.bin> <leaf> <bucket> <out.bin>
 //                                                   constructor + operator() + operator<< (GPU)
 // keys.bin holds n little-endian (first, second) uint64 pairs.
@@ -54,6 +57,34 @@ static int run(int argc, char **argv) {
         std::printf("load ok: n=%zu bits/key=%.4f\n", keys.size(), (double)rs.getBits() / keys.size());
         return 0;
     }
+    if (mode == "loadstr" || mode == "buildstr") {
+        std::vector<std::string> keys;
+        {
+            std::ifstream f(mode == "loadstr" ? argv[3] : argv[2]);
+            for (std::string k; std::getline(f, k);) keys.push_back(k);
+        }
+        GPURecSplit<L> rs;
+        if (mode == "loadstr") {
+            std::ifstream in(argv[2], std::ios::binary);
+            in >> rs;
+        } else {
+            std::ifstream in(argv[2]);
+            rs = GPURecSplit<L>(in, std::stoul(argv[4]));
+            std::ofstream out(argv[5], std::ios::binary);
+            out << rs;
+        }
+        std::vector<bool> taken(keys.size(), false);
+        for (const auto &k : keys) {
+            const size_t v = rs(k);
+            if (v >= keys.size() || taken[v]) {
+                std::printf("collision or out of range for key %s: %zu\n", k.c_str(), v);
+                return 1;
+            }
+            taken[v] = true;
+        }
+        std::printf("%s ok: n=%zu\n", mode.c_str(), keys.size());
+        return 0;
+    }
     if (mode == "load2") {
         GPURecSplit<L> a, b;
         std::ifstream in(argv[2], std::ios::binary);
@@ -85,7 +116,7 @@ int main(int argc, char **argv) {
         return 2;
     }
     const std::string mode = argv[1];
-    const int leaf = std::stoi(mode == "load" ? argv[4] : mode == "load2" ? argv[5] : argv[3]);
+    const int leaf = std::stoi(mode == "load" || mode == "loadstr" ? argv[4] : mode == "load2" ? argv[5] : argv[3]);
     try {
         switch (leaf) {
         case 8: return run<8>(argc, argv);

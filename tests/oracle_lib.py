@@ -32,6 +32,11 @@ def _load():
         "orc_search_aggr": (C.c_uint64, [_u64p, C.c_uint32, C.c_uint32, C.c_uint64, _u64p]),
         "orc_search_leaf_rot": (C.c_uint64, [_u64p, C.c_uint32, _u64p]),
         "orc_search_leaf_brute": (C.c_uint64, [_u64p, C.c_uint32, _u64p]),
+        "orc_remap64": (C.c_uint32, [C.c_uint64, C.c_uint32]),
+        "orc_search_higher_r64": (C.c_uint64, [_u64p, C.c_uint32, C.c_uint32, C.c_int, _u64p]),
+        "orc_search_aggr_r64": (C.c_uint64, [_u64p, C.c_uint32, C.c_uint32, C.c_uint64, _u64p]),
+        "orc_search_leaf_rot_r64": (C.c_uint64, [_u64p, C.c_uint32, _u64p]),
+        "orc_search_leaf_brute_r64": (C.c_uint64, [_u64p, C.c_uint32, _u64p]),
         "orc_packed_aggr_predicate": (C.c_int, [_u32p, C.c_uint32, C.c_uint32]),
         "orc_search_form_aggr_predicate": (C.c_int, [_u32p, C.c_uint32, C.c_uint32]),
         "orc_build": (C.c_void_p, [_u64p, C.c_size_t, C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_int)]),
@@ -104,8 +109,14 @@ ERRORS = {-1: "bad arguments", -2: "bucket with >= 3000 keys", -3: "equal second
           -4: "Elias-Fano jump offset overflow"}
 
 
+# flags of the `rotate` argument: rotation fitting for full leaves, 64-bit remap
+# (the RecSplitRotate variant)
+FLAG_ROTATE, FLAG_REMAP64 = 1, 2
+
+
 class OracleMPHF:
-    """Oracle MPHF: build from keys, or load from serialized bytes."""
+    """Oracle MPHF: build from keys, or load from serialized bytes.  `rotate` is a
+    bool or a flags word (FLAG_ROTATE | FLAG_REMAP64)."""
 
     def __init__(self, handle, leaf: int, rotate: bool):
         self.h = handle

@@ -284,6 +284,97 @@ def test_packed_counters_carry_cases():
                     assert _packed(parts, m, split) == _exact(c, m, split)
 
 
+# --------------------------------------------------------------------------- 64-bit remap
+
+
+def test_remap64_is_48_bit_lemire_of_remix():
+    """The RecSplitRotate remap (AbstractParallelRecSplit.hpp:327-332): the low 48
+    bits of the 64-bit finalizer (pinned by the SplitMix64 outputs above), scaled
+    to [0, n): n = 1 gives 0, n = 2^16 gives bits 32..47 of the finalizer, and any
+    n matches 128-bit arithmetic."""
+    rng = np.random.default_rng(2)
+    for x in rng.integers(0, 2**63, size=200, dtype=np.uint64):
+        x = int(x) * 2 + 1
+        r = lib.orc_remix(x & M64)
+        assert lib.orc_remap64(x & M64, 1) == 0
+        assert lib.orc_remap64(x & M64, 65536) == (r >> 32) & 0xFFFF
+        for n in (2, 3, 24, 100, 2999):
+            assert lib.orc_remap64(x & M64, n) == ((r & ((1 << 48) - 1)) * n) >> 48
+
+
+def _remap64(x, n):
+    return lib.orc_remap64(x & M64, n)
+
+
+def test_r64_searches_minimal():
+    """Every node search in 64-bit-remap mode returns the brute-force lowest seed
+    (the same predicates as above, with the 64-bit remap)."""
+    S19 = START_SEED[19]
+    for trial in range(24):
+        m, split, level = random.Random(trial).choice([(6, 3), (8, 4), (12, 8), (5, 2)]) + (trial % 4,)
+        keys = _keys(500 + trial, m)
+        got = lib.orc_search_higher_r64(ptr64(keys), m, split, level, None)
+        x = 0
+        while sum(_remap64(int(k) + START_SEED[level] + x, m) < split for k in keys) != split:
+            x += 1
+        assert got == x
+        m, split = random.Random(trial).choice([(6, 2), (9, 3), (10, 4), (11, 2)])
+        seed = START_SEED[17 + trial % 2]
+        keys = _keys(600 + trial, m)
+        got = lib.orc_search_aggr_r64(ptr64(keys), m, split, seed, None)
+        f = (m + split - 1) // split
+        target = [split] * (f - 1) + [m - (f - 1) * split]
+        x = 0
+        while True:
+            c = [0] * f
+            for k in keys:
+                c[_remap64(int(k) + seed + x, m) // split] += 1
+            if c == target:
+                break
+            x += 1
+        assert got == x
+        m = 2 + trial % 7
+        keys = _keys(700 + trial, m)
+        got = lib.orc_search_leaf_brute_r64(ptr64(keys), m, None)
+        x = 0
+        while len({_remap64(int(k) + S19 + x, m) for k in keys}) != m:
+            x += 1
+        assert got == x
+        L = 3 + trial % 6
+        keys = _keys(800 + trial, L)
+        got = lib.orc_search_leaf_rot_r64(ptr64(keys), L, None)
+        s = 0
+        while True:
+            base, r = s - s % L, s % L
+            ml = mr = 0
+            for k in keys:
+                bit = 1 << _remap64(int(k) + S19 + base, L)
+                if int(k) % 2 == 0:
+                    ml |= bit
+                else:
+                    mr |= bit
+            if ml | _rotl(mr, r, L) == (1 << L) - 1:
+                break
+            s += 1
+        assert got == s, (L, trial)
+
+
+@pytest.mark.parametrize("leaf,bucket,n", [(2, 7, 1000), (5, 10, 3000), (8, 100, 10000), (12, 100, 8000),
+                                           (16, 2000, 4000), (18, 50, 1500)])
+def test_r64_mphf_bijection_and_differs(leaf, bucket, n):
+    from oracle_lib import FLAG_REMAP64, FLAG_ROTATE
+    keys = splitmix_keys(leaf * 77 + bucket, n)
+    rs = OracleMPHF.build(keys, leaf, bucket, rotate=FLAG_ROTATE | FLAG_REMAP64)
+    assert rs.bijection_errors(keys) == 0
+    back = OracleMPHF.load(rs.serialize(), leaf, FLAG_ROTATE | FLAG_REMAP64)
+    assert np.array_equal(back.query(keys), rs.query(keys))
+    if n > 100:
+        assert rs.serialize() != OracleMPHF.build(keys, leaf, bucket, rotate=True).serialize()
+    if leaf <= 8:
+        rs2 = OracleMPHF.build(keys, leaf, bucket, rotate=FLAG_REMAP64)
+        assert rs2.bijection_errors(keys) == 0
+
+
 # --------------------------------------------------------------------------- whole MPHF
 
 
