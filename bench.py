@@ -159,7 +159,7 @@ def batched_4x5m(gra, ctx, args, plain_ms: float) -> dict:
             "search_share": sum(st["search_ms"].values()) / st["total_ms"]}
 
 
-def batched_4096(gra, ctx, quick: bool) -> dict:
+def batched_4096(gra, ctx, quick: bool, sequential: bool = True) -> dict:
     """4096 sets, per-set sizes log-uniform in [1e3, 1e6] (numpy default_rng(3)),
     keys from one SplitMix64(3) stream generated in HBM, l = 12, b = 100: one
     batch, then the same sets as 4096 plain builds one after another.  Every set is
@@ -192,14 +192,16 @@ def batched_4096(gra, ctx, quick: bool) -> dict:
     oracle_ok = all(rss[i].serialize() == OracleMPHF.build(d[int(off[i]):int(off[i + 1])].cpu().numpy().view(np.uint64),
                                                            12, 100, threads=16).serialize() for i in sample)
     del rss
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(4096):
-        rs = gra.GPURecSplit(d[int(off[i]):int(off[i + 1])], 100, leaf_size=12, ctx=ctx)
-        del rs
-    torch.cuda.synchronize()
-    seq_ms = (time.perf_counter() - t0) * 1e3
-    log(f"4096 sets one by one: {seq_ms:.1f} ms")
+    seq_ms = float("nan")
+    if sequential:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(4096):
+            rs = gra.GPURecSplit(d[int(off[i]):int(off[i + 1])], 100, leaf_size=12, ctx=ctx)
+            del rs
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3
+        log(f"4096 sets one by one: {seq_ms:.1f} ms")
     return {"sets": 4096, "keys": n, "leaf": 12, "bucket": 100, "sizes": f"log-uniform [1e3, {hi:.0e}]",
             "ms": ms, "value": ms * 1e6 / n, "unit": "ns/object", "search_ms": st["search_ms"],
             "partition_ms": st["partition_ms"], "encode_ms": st["encode_ms"],
@@ -264,6 +266,10 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=16.0, help="job arrivals per second of the served stream")
     ap.add_argument("--serve-time-limit", type=float, default=120.0, help="per-batch build time limit (s)")
     ap.add_argument("--no-4096", action="store_true", help="skip the 4096-set batched workload")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched4096: skip the 4096 one-by-one builds (profiling runs)")
+    ap.add_argument("--only", choices=["l18", "l24", "batched4x5M", "batched4096", "served"],
+                    help="run one extra workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -278,6 +284,21 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 1
     ctx = gra.Context(0)
+    if args.only:
+        if args.only == "l18":
+            d = splitmix64_keys_device(2, 5_000_000)
+            out = plain(gra, ctx, d, 5_000_000, 18, 50, args.steps, args.warmup, "l18 b50")
+        elif args.only == "l24":
+            d = splitmix64_keys_device(24, 40_000)
+            out = plain(gra, ctx, d, 40_000, 24, 2000, args.steps, args.warmup, "l24 b2000")
+        elif args.only == "batched4x5M":
+            out = batched_4x5m(gra, ctx, args, float("nan"))
+        elif args.only == "batched4096":
+            out = batched_4096(gra, ctx, quick=False, sequential=not args.no_sequential)
+        else:
+            out = served(args)
+        print(json.dumps({"only": args.only, **out}))
+        return 0
     d_keys = splitmix64_keys_device(args.seed, args.n)
     torch.cuda.synchronize()
     log(f"keys resident: n={args.n} l={args.leaf} b={args.bucket}")

@@ -15,8 +15,10 @@ import fails loudly when the HIP library is missing.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -33,6 +35,14 @@ STATUS = {
 }
 
 KIND_NAMES = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforce"]
+
+# Flags word passed as the C ABI's `rotate` argument.
+FLAG_ROTATE = 1    # rotation fitting for full leaves (USE_BIJECTIONS_ROTATE)
+FLAG_REMAP64 = 2   # 64-bit remap of the RecSplitRotate variant (USE_64_BIT_REMAP)
+
+
+def _flags(rotate: bool, remap64: bool) -> int:
+    return (FLAG_ROTATE if rotate else 0) | (FLAG_REMAP64 if remap64 else 0)
 
 
 class RecSplitError(RuntimeError):
@@ -88,6 +98,11 @@ def _load():
         "rs_mphf_query": (C.c_uint64, [vp, C.c_uint64, C.c_uint64]),
         "rs_mphf_query_batch": (None, [vp, u64p, C.c_size_t, u64p]),
         "rs_query_device": (C.c_int, [vp, vp, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "rs_autotune": (C.c_int, [vp, C.c_uint32, C.c_size_t, C.c_int, C.c_uint64, C.c_char_p, C.c_size_t]),
+        "rs_ctx_set_tune": (C.c_int, [vp, C.c_uint32, C.c_char_p]),
+        "rs_ctx_get_tune": (C.c_int, [vp, C.c_uint32, C.c_char_p, C.c_size_t]),
+        "rs_ctx_device_bytes": (C.c_uint64, [vp]),
+        "rs_ctx_release": (C.c_int, [vp]),
         "rs_build_device_noalloc": (C.c_int, [vp, C.c_void_p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
                                               C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "rs_build_batch": (C.c_int, [vp, C.c_void_p, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
@@ -114,6 +129,19 @@ def _load():
 
 
 lib = _load()
+
+# Every live context, structure and server, released in a fixed order (servers,
+# then structures, then contexts) by an exit hook that runs before interpreter
+# teardown -- while the HIP runtime and torch are still fully alive -- instead of
+# by __del__ calls in whatever order module teardown reaches them.
+_live = {"server": weakref.WeakSet(), "mphf": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+
+
+@atexit.register
+def _release_all():
+    for kind in ("server", "mphf", "ctx"):
+        for obj in list(_live[kind]):
+            obj.close()
 
 
 def _check(code: int):
@@ -169,6 +197,7 @@ class Context:
         _check(lib.rs_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        _live["ctx"].add(self)
         if time_limit_s is not None:
             self.set_time_limit(time_limit_s)
 
@@ -176,6 +205,29 @@ class Context:
         """Bounds every later build on this context (0: no limit); a build past
         the limit raises RecSplitError with code 10 and the context stays usable."""
         _check(lib.rs_ctx_set_time_limit(self.h, float(seconds)))
+
+    def autotune(self, leaf_size: int, bucket_size: int, rotate: bool = True, remap64: bool = False,
+                 sample_keys: int = 0) -> dict:
+        """Times synthetic builds of this leaf size and bucket size under candidate
+        launch settings and keeps the fastest as the context's tuning for the leaf
+        size; returns the tuner's report."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib.rs_autotune(self.h, leaf_size, bucket_size, _flags(rotate, remap64), sample_keys, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def set_tune(self, tune: str, leaf_size: int = 0):
+        """Applies a tuning string (RS_TUNE syntax) to a leaf size (0: all)."""
+        _check(lib.rs_ctx_set_tune(self.h, leaf_size, tune.encode()))
+
+    def get_tune(self, leaf_size: int = 0) -> str:
+        buf = C.create_string_buffer(1024)
+        _check(lib.rs_ctx_get_tune(self.h, leaf_size, buf, len(buf)))
+        return buf.value.decode()
+
+    def device_bytes(self) -> int:
+        """Device memory the context holds for reuse by later builds."""
+        return int(lib.rs_ctx_device_bytes(self.h))
 
     @classmethod
     def default(cls, device: int = 0) -> "Context":
@@ -203,13 +255,15 @@ class GPURecSplit:
     """
 
     def __init__(self, keys=None, bucket_size: int = 100, leaf_size: int = 8, rotate: bool = True,
-                 num_threads: int = 1, ctx: Context | None = None, _handle=None):
+                 num_threads: int = 1, ctx: Context | None = None, _handle=None, remap64: bool = False):
         self.leaf_size = leaf_size
         self.rotate = rotate
+        self.remap64 = remap64
         self.bucket_size = bucket_size
         self._ctx = ctx
         if _handle is not None:
             self.h = _handle
+            _live["mphf"].add(self)
             return
         self._ctx = ctx or Context.default()
         h = C.c_void_p()
@@ -218,23 +272,24 @@ class GPURecSplit:
                 raise ValueError("device keys must be a contiguous (n, 2) 64-bit tensor")
             _sync_torch(keys)
             _check(lib.rs_build_device(self._ctx.h, C.c_void_p(keys.data_ptr()), keys.shape[0], leaf_size,
-                                       bucket_size, int(rotate), C.byref(h)))
+                                       bucket_size, _flags(rotate, remap64), C.byref(h)))
         else:
             k = _keys_array(keys)
             _check(lib.rs_build(self._ctx.h, k.ctypes.data_as(C.POINTER(C.c_uint64)), k.shape[0], leaf_size,
-                                bucket_size, int(rotate), C.byref(h)))
+                                bucket_size, _flags(rotate, remap64), C.byref(h)))
         self.h = h
+        _live["mphf"].add(self)
 
     @classmethod
     def from_strings(cls, keys, bucket_size: int, leaf_size: int = 8, rotate: bool = True,
-                     ctx: Context | None = None) -> "GPURecSplit":
+                     ctx: Context | None = None, remap64: bool = False) -> "GPURecSplit":
         """GPURecSplit(const vector<string>&, bucket_size, numThreads) (GPURecSplit.cuh:443-478):
         string or bytes keys, hashed with SpookyHash V2 on the GPU."""
         ctx = ctx or Context.default()
         blob, off = _pack_strings(keys)
         h = C.c_void_p()
         _check(lib.rs_build_strings(ctx.h, blob, off.ctypes.data_as(C.POINTER(C.c_uint64)), len(off) - 1, leaf_size,
-                                    bucket_size, int(rotate), C.byref(h)))
+                                    bucket_size, _flags(rotate, remap64), C.byref(h)))
         return cls(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=h)
 
     def query_string(self, key) -> int:
@@ -244,12 +299,14 @@ class GPURecSplit:
 
     # operator>> (GPURecSplit.cuh:827-841)
     @classmethod
-    def deserialize(cls, data: bytes, leaf_size: int, rotate: bool = True) -> "GPURecSplit":
+    def deserialize(cls, data: bytes, leaf_size: int, rotate: bool = True, remap64: bool = False) -> "GPURecSplit":
+        """The serialized layout does not record the rotation or remap setting: pass
+        the ones the structure was built with (the reference fixes them by type)."""
         h = C.c_void_p()
-        _check(lib.rs_mphf_deserialize(data, len(data), leaf_size, int(rotate), C.byref(h)))
+        _check(lib.rs_mphf_deserialize(data, len(data), leaf_size, _flags(rotate, remap64), C.byref(h)))
         import struct
         bucket = struct.unpack_from("<Q", data, 8)[0]
-        return cls(bucket_size=bucket, leaf_size=leaf_size, rotate=rotate, _handle=h)
+        return cls(bucket_size=bucket, leaf_size=leaf_size, rotate=rotate, _handle=h, remap64=remap64)
 
     # operator<< (GPURecSplit.cuh:817-825)
     def serialize(self) -> bytes:
@@ -297,14 +354,18 @@ class GPURecSplit:
         _check(lib.rs_mphf_stats(self.h, C.byref(s)))
         return s.as_dict()
 
-    def __del__(self, _free=lib.rs_mphf_free):
+    def close(self, _free=lib.rs_mphf_free):
+        """Frees the structure (host and device copies); later calls do nothing."""
         if getattr(self, "h", None):
             _free(self.h)
             self.h = None
 
+    def __del__(self):
+        self.close()
+
 
 def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: bool = True,
-                ctx: Context | None = None):
+                ctx: Context | None = None, remap64: bool = False):
     """Builds one MPHF per key set in one batch on the GPU.
 
     d_keys: contiguous (n, 2) 64-bit CUDA tensor holding all sets back to back;
@@ -322,14 +383,14 @@ def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: b
     handles = (C.c_void_p * S)()
     st = BuildStats()
     _check(lib.rs_build_batch(ctx.h, C.c_void_p(d_keys.data_ptr()), off.ctypes.data_as(C.POINTER(C.c_uint64)), S,
-                              leaf_size, bucket_size, int(rotate), handles, C.byref(st)))
-    out = [GPURecSplit(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=C.c_void_p(h))
-           for h in handles]
+                              leaf_size, bucket_size, _flags(rotate, remap64), handles, C.byref(st)))
+    out = [GPURecSplit(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=C.c_void_p(h),
+                       remap64=remap64) for h in handles]
     return out, st.as_dict()
 
 
 def build_noalloc(d_keys, bucket_size: int, leaf_size: int, rotate: bool = True, ctx: Context | None = None,
-                  out_capacity: int | None = None) -> bytes:
+                  out_capacity: int | None = None, remap64: bool = False) -> bytes:
     """Allocation-free build (rs_build_device_noalloc): the serialized MPHF of the
     keys in d_keys (a contiguous (n, 2) 64-bit CUDA tensor), built entirely in the
     library's fixed per-GPU scratch array.  out_capacity bounds the host output
@@ -342,6 +403,7 @@ def build_noalloc(d_keys, bucket_size: int, leaf_size: int, rotate: bool = True,
     buf = C.create_string_buffer(cap)
     ln = C.c_size_t(0)
     _sync_torch(d_keys)
-    _check(lib.rs_build_device_noalloc(ctx.h, C.c_void_p(d_keys.data_ptr()), n, leaf_size, bucket_size, int(rotate),
+    _check(lib.rs_build_device_noalloc(ctx.h, C.c_void_p(d_keys.data_ptr()), n, leaf_size, bucket_size,
+                                       _flags(rotate, remap64),
                                        buf, cap, C.byref(ln)))
     return buf.raw[: ln.value]

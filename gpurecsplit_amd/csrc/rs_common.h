@@ -83,4 +83,23 @@ RS_HD uint32_t fold_h16(uint64_t z) {
 
 RS_HD uint32_t remap16(uint64_t z, uint32_t n) { return (fold_h16(z) * n) >> 16; }
 
+// Flags word of a build (the `rotate` argument of the C ABI): rotation fitting for
+// full leaves (USE_BIJECTIONS_ROTATE), and the 64-bit remap of the RecSplitRotate
+// variant (USE_64_BIT_REMAP, RecSplitRotate.hpp:115).
+constexpr int kFlagRotate = 1, kFlagRemap64 = 2;
+
+// remixAndRemap with USE_64_BIT_REMAP (AbstractParallelRecSplit.hpp:327-332): the
+// 64-bit Stafford finalizer (remix, :265-269), its low 48 bits scaled to [0, n).
+// With hi16 and lo the bits 32..47 and 0..31 of the finalizer:
+// (hi16 * 2^32 + lo) * n >> 48 = (hi16 * n + floor(lo * n / 2^32)) >> 16.
+RS_HD uint32_t remap64(uint64_t z, uint32_t n) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    const uint32_t lo = (uint32_t)z, hi16 = (uint32_t)(z >> 32) & 0xFFFFu;
+    return (hi16 * n + (uint32_t)(((uint64_t)lo * n) >> 32)) >> 16;
+}
+
+RS_HD uint32_t remap_any(uint64_t z, uint32_t n, bool r64) { return r64 ? remap64(z, n) : remap16(z, n); }
+
 }  // namespace rsa

@@ -230,8 +230,7 @@ struct Tune {
     int blocks_per_cu = 8;
 };
 
-static Tune parse_tune(const char *env) {
-    Tune t;
+static Tune parse_tune(const char *env, Tune t = Tune()) {
     if (!env) return t;
     std::string s(env);
     size_t p = 0;
@@ -264,6 +263,19 @@ static Tune parse_tune(const char *env) {
     return t;
 }
 
+// The same settings as an RS_TUNE string (every field).
+static std::string format_tune(const Tune &t) {
+    std::string s;
+    auto add = [&](const std::string &k, int v) { s += (s.empty() ? "" : ",") + k + "=" + std::to_string(v); };
+    for (int k = 0; k < kNumKinds; k++) add("u" + std::to_string(k), t.u[k]);
+    for (int k = 0; k < kNumKinds; k++) add("g" + std::to_string(k), t.groups[k]);
+    for (int k = 0; k < kNumKinds; k++) add("sk" + std::to_string(k), t.sk[k]);
+    for (int k = 0; k < kNumKinds; k++) add("h" + std::to_string(k), t.help_cap[k]);
+    add("lut", t.lut);
+    add("bpc", t.blocks_per_cu);
+    return s;
+}
+
 // Scratch of the allocation-free build (rs_build_device_noalloc): a fixed array in
 // the library's code object, one copy per GPU, held until the process ends.  Host
 // code never passes the array's name to a kernel: each device's address comes
@@ -277,7 +289,10 @@ static uint8_t *g_scratch_ptr[kMaxDevices] = {};
 static std::mutex g_scratch_use[kMaxDevices];  // one allocation-free build per GPU at a time
 
 struct rs_ctx {
-    Tune tune;
+    Tune tune;                      // every leaf size without its own tuning
+    Tune tune_leaf[kMaxLeaf + 1];   // per leaf size, from rs_autotune / rs_ctx_set_tune
+    bool tuned[kMaxLeaf + 1] = {};
+    bool autotune = false;          // RS_AUTOTUNE: tune each leaf size at its first build
     int device = 0;
     uint8_t *scratch = nullptr;  // this device's address of `scratch`
     int num_cus = 0;
@@ -285,7 +300,7 @@ struct rs_ctx {
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
         list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
-        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock, str_bytes, str_off;
+        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock, str_bytes, str_off, tune_keys;
     double time_limit_s = 0;  // 0: none
     int wall_khz = 0;         // device wall-clock rate
     uint32_t *memo[kMaxLeaf + 1] = {};
@@ -382,45 +397,49 @@ static double ev_ms(rs_ctx *c, int a, int b) {
     return ms;
 }
 
-template <int KIND, int U, bool SK>
+template <int KIND, int U, bool SK, bool R64>
 static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
                         size_t lds) {
     hipStream_t s = c->stream;
     if (KIND == kUpper || KIND == kLower) {
-        if (wide) k_search<KIND, U, true, false, SK><<<blocks, 256, 0, s>>>(keys, a);
-        else if (lut) k_search<KIND, U, false, true, SK><<<blocks, 256, lds, s>>>(keys, a);
-        else k_search<KIND, U, false, false, SK><<<blocks, 256, 0, s>>>(keys, a);
+        if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
+        else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
+        else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
+        else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     } else {
-        k_search<KIND, U, false, false, SK><<<blocks, 256, 0, s>>>(keys, a);
+        k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     }
 }
 
-template <int U, bool SK>
+template <int U, bool SK, bool R64>
 static void launch_u(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                      uint32_t blocks, size_t lds) {
     switch (kind) {
-    case kHigher: launch_kind<kHigher, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
-    case kUpper: launch_kind<kUpper, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLower: launch_kind<kLower, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLeafRot: launch_kind<kLeafRot, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
-    default: launch_kind<kLeafBrute, U, SK>(c, keys, a, wide, lut, blocks, lds); break;
+    case kHigher: launch_kind<kHigher, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
+    case kUpper: launch_kind<kUpper, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLower: launch_kind<kLower, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
+    case kLeafRot: launch_kind<kLeafRot, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
+    default: launch_kind<kLeafBrute, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
     }
 }
 
-template <bool SK>
-static void launch_sk(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
+template <bool SK, bool R64>
+static void launch_sk(rs_ctx *c, int u, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                       uint32_t blocks, size_t lds) {
-    switch (c->tune.u[kind]) {
-    case 1: launch_u<1, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    case 4: launch_u<4, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    default: launch_u<2, SK>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    switch (u) {
+    case 1: launch_u<1, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    case 4: launch_u<4, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    default: launch_u<2, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
     }
 }
 
-static void launch_search(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
-                          uint32_t blocks, size_t lds) {
-    if (c->tune.sk[kind]) launch_sk<true>(c, kind, keys, a, wide, lut, blocks, lds);
-    else launch_sk<false>(c, kind, keys, a, wide, lut, blocks, lds);
+// The 64-bit remap runs with keys in scalar registers only.
+static void launch_search(rs_ctx *c, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
+                          bool wide, bool lut, uint32_t blocks, size_t lds, bool r64) {
+    const int u = tn.u[kind];
+    if (r64) launch_sk<true, true>(c, u, kind, keys, a, wide, lut, blocks, lds);
+    else if (tn.sk[kind]) launch_sk<true, false>(c, u, kind, keys, a, wide, lut, blocks, lds);
+    else launch_sk<false, false>(c, u, kind, keys, a, wide, lut, blocks, lds);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -431,6 +450,8 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
 }
 
 constexpr uint64_t kTileKeys = 1 << 14;  // keys per histogram / scatter workgroup
+
+static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags, uint64_t sample_keys);
 
 // Where a single-set build writes its serialized structure instead of returning a
 // handle (the allocation-free entry point): the caller's buffer.
@@ -448,9 +469,11 @@ struct SerialOut {
 // is made.
 static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint64_t *> &set_keys,
                                          const std::vector<uint64_t> &set_n, uint32_t leaf, uint64_t bucket_size,
-                                         bool rotate, bool device_copy,
+                                         int flags, bool device_copy,
                                          const std::chrono::steady_clock::time_point t_start,
                                          const SerialOut *so = nullptr) {
+    if (c->autotune && !c->tuned[leaf] && !t_arena) autotune(c, leaf, bucket_size, flags, 0);
+    const bool rotate = (flags & kFlagRotate) != 0, r64 = (flags & kFlagRemap64) != 0;
     std::vector<uint64_t> set_off(set_n.size() + 1, 0);  // key offsets of the sets in the sorted array
     for (size_t i = 0; i < set_n.size(); i++) set_off[i + 1] = set_off[i] + set_n[i];
     hipStream_t s = c->stream;
@@ -568,7 +591,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     uint64_t max_list = 1;
     for (int l = 0; l < kNumLists; l++) max_list = std::max(max_list, cen.count[l]);
     NodeState *state = c->state.get<NodeState>(max_list);
-    const Tune &tn = c->tune;
+    const Tune &tn = c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune;
     const uint32_t blocks = (uint32_t)c->num_cus * (uint32_t)tn.blocks_per_cu;
     const uint32_t n_waves = blocks * 4;
     st.search_waves = n_waves;
@@ -606,7 +629,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.err = err;
         a.groups = (uint32_t)tn.groups[kind];
         a.help_cap = (uint32_t)tn.help_cap[kind];
-        bool wide = false;
+        bool wide = false, lut = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
             const AggrLayout &L = kind == kUpper ? up_l : lo_l;
@@ -614,15 +637,20 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             a.field_bits = L.fb;
             a.magic = L.magic;
             wide = L.wide;
-            a.lut_stride = ((kind == kUpper ? P.upper : P.lower) + 3) & ~3u;
+            // one increment table per wave: 4-byte entries, 8-byte ones for 64-bit
+            // counters; tables above 32 KiB per workgroup would cut occupancy
+            const uint32_t entries = ((kind == kUpper ? P.upper : P.lower) + 3) & ~3u;
+            a.lut_stride = entries * (wide ? 2 : 1);
             lds = (size_t)4 * a.lut_stride * 4;
+            lut = tn.lut != 0 && lds <= 32768;
+            if (!lut) lds = 0;
         }
         PhaseEvents pe{kind, ev_record(c), -1, -1};
-        launch_search(c, kind, sorted, a, wide, tn.lut != 0, blocks, lds);
+        launch_search(c, tn, kind, sorted, a, wide, lut, blocks, lds, r64);
         pe.search1 = ev_record(c);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);
-            k_split<<<cnt, 64, (size_t)max_m * 8, s>>>(sorted, nodes, state, seeds, err, P, kind);
+            k_split<<<cnt, 64, (size_t)max_m * 8, s>>>(sorted, nodes, state, seeds, err, P, kind, r64 ? 1 : 0);
         } else {
             k_finalize<<<(cnt + 255) / 256, 256, 0, s>>>(nodes, state, cnt, seeds, err);
         }
@@ -825,7 +853,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     for (size_t i = 0; i < S; i++) {
         rs_mphf *r = new rs_mphf();
         r->leaf = leaf;
-        r->rotate = rotate ? 1 : 0;
+        r->rotate = flags & (kFlagRotate | kFlagRemap64);
         r->bucket_size = bucket_size;
         r->keys_count = set_off[i + 1] - set_off[i];
         r->nbuckets = nbs[i];
@@ -849,8 +877,8 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
 }
 
 static rs_mphf *build(rs_ctx *c, const uint64_t *d_keys, uint64_t n, uint32_t leaf, uint64_t bucket_size,
-                      bool rotate, const std::chrono::steady_clock::time_point t_start) {
-    return build_sets(c, {d_keys}, {n}, leaf, bucket_size, rotate, true, t_start)[0];
+                      int flags, const std::chrono::steady_clock::time_point t_start) {
+    return build_sets(c, {d_keys}, {n}, leaf, bucket_size, flags, true, t_start)[0];
 }
 
 // Every reusable device buffer of a context.
@@ -859,7 +887,112 @@ static std::vector<DevBuf *> ctx_bufs(rs_ctx *c) {
             &c->size_hist, &c->scan_tmp, &c->sorted, &c->pool, &c->list_base, &c->list_fill, &c->node_m,
             &c->seeds, &c->state, &c->running, &c->ctl, &c->err, &c->bits, &c->pos, &c->evals, &c->work, &c->rice,
             &c->ef_scalars, &c->ef_lower, &c->ef_uc, &c->ef_up, &c->ef_jump, &c->tiles, &c->set_idx,
-            &c->set_gather, &c->set_enc, &c->bit_off, &c->clock, &c->str_bytes, &c->str_off};
+            &c->set_gather, &c->set_enc, &c->bit_off, &c->clock, &c->str_bytes, &c->str_off, &c->tune_keys};
+}
+
+// SplitMix64 keys on the device (the generator of the synthetic workloads): key i
+// is (next(), next()) of the stream from `seed`.
+__global__ __launch_bounds__(256) void k_splitmix_keys(uint64_t seed, uint64_t n, uint64_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        uint64_t z = seed + 0x9e3779b97f4a7c15ULL * (2 * i + 1 + j);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        out[2 * i + j] = z ^ (z >> 31);
+    }
+}
+
+// Autotuner: picks, per node kind, the candidates per lane (U) and candidate
+// groups per claimed chunk, then the workgroups per CU, by timing builds of a
+// synthetic sample (SplitMix64 keys) of this leaf size, bucket size and flags.
+// Every kind has its own search launch, so one trial measures all kinds at once;
+// the sample is sized so that a build takes about 40 ms.  The result becomes
+// the context's tuning for the leaf size.  Returns a JSON report.
+static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags, uint64_t sample_keys) {
+    const Tune base = c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune;
+    c->tuned[leaf] = true;  // builds below run with explicit tunings (no recursion)
+    const uint64_t n_max = uint64_t(1) << 22;
+    uint64_t *keys = c->tune_keys.get<uint64_t>(2 * n_max);
+    k_splitmix_keys<<<(n_max + 255) / 256, 256, 0, c->stream>>>(0x5eed, n_max, keys);
+    HIP_CHECK(hipGetLastError());
+    auto trial = [&](const Tune &t, uint64_t n) {
+        c->tune_leaf[leaf] = t;
+        std::vector<rs_mphf *> r = build_sets(c, {keys}, {n}, leaf, bucket, flags, false, std::chrono::steady_clock::now());
+        const rs_build_stats st = r[0]->stats;
+        for (rs_mphf *h : r) rs_mphf_free(h);
+        return st;
+    };
+    uint64_t n = sample_keys ? std::min(sample_keys, n_max) : std::max<uint64_t>(2048, bucket);
+    rs_build_stats st0 = trial(base, n);
+    double total0 = 0;
+    for (int k = 0; k < kNumKinds; k++) total0 += st0.search_ms[k];
+    if (!sample_keys) {
+        const double scale = total0 > 0 ? 40.0 / total0 : 64.0;
+        n = std::min<uint64_t>(n_max, (uint64_t)(n * std::min(64.0, std::max(1.0, scale))));
+        st0 = trial(base, n);
+    }
+    const int reps = st0.total_ms > 100 ? 1 : 2;
+    auto timed = [&](const Tune &t) {
+        rs_build_stats best = trial(t, n);
+        for (int r = 1; r < reps; r++) {
+            const rs_build_stats s2 = trial(t, n);
+            for (int k = 0; k < kNumKinds; k++) best.search_ms[k] = std::min(best.search_ms[k], s2.search_ms[k]);
+        }
+        return best;
+    };
+    auto total = [](const rs_build_stats &st) {
+        double t = 0;
+        for (int k = 0; k < kNumKinds; k++) t += st.search_ms[k];
+        return t;
+    };
+    const rs_build_stats ref = timed(base);
+    std::string rep = "{\"leaf\":" + std::to_string(leaf) + ",\"bucket\":" + std::to_string(bucket) +
+                      ",\"flags\":" + std::to_string(flags) + ",\"sample_keys\":" + std::to_string(n) +
+                      ",\"search_ms_before\":" + std::to_string(total(ref)) + ",\"trials\":[";
+    Tune best = base;
+    double best_ms[kNumKinds];
+    for (int k = 0; k < kNumKinds; k++) best_ms[k] = ref.search_ms[k];
+    const int us[3] = {1, 2, 4}, gs[3] = {1, 2, 4};
+    bool first = true;
+    for (int u : us)
+        for (int g : gs) {
+            Tune t = base;
+            for (int k = 0; k < kNumKinds; k++) {
+                t.u[k] = u;
+                t.groups[k] = g;
+            }
+            const rs_build_stats st = timed(t);
+            rep += std::string(first ? "" : ",") + "{\"u\":" + std::to_string(u) + ",\"g\":" + std::to_string(g) +
+                   ",\"search_ms\":[";
+            for (int k = 0; k < kNumKinds; k++) {
+                rep += (k ? "," : "") + std::to_string(st.search_ms[k]);
+                if (st.nodes[k] && st.search_ms[k] < best_ms[k]) {
+                    best_ms[k] = st.search_ms[k];
+                    best.u[k] = u;
+                    best.groups[k] = g;
+                }
+            }
+            rep += "]}";
+            first = false;
+        }
+    double best_total = total(timed(best));
+    const int bpcs[3] = {6, 8, 10};
+    for (int b : bpcs) {
+        if (b == best.blocks_per_cu) continue;
+        Tune t = best;
+        t.blocks_per_cu = b;
+        const double ms = total(timed(t));
+        rep += ",{\"bpc\":" + std::to_string(b) + ",\"search_ms_total\":" + std::to_string(ms) + "}";
+        if (ms < best_total) {
+            best_total = ms;
+            best.blocks_per_cu = b;
+        }
+    }
+    c->tune_leaf[leaf] = best;
+    rep += "],\"search_ms_after\":" + std::to_string(best_total) + ",\"tune\":\"" + format_tune(best) + "\"}";
+    return rep;
 }
 
 static int guarded(const char *what, rs_ctx *c, const std::function<void()> &fn) {
@@ -933,9 +1066,9 @@ static int check_args(rs_ctx *c, const void *keys, size_t n, uint32_t leaf, size
 
 // Hooks of the construction server (rs_server.cpp).
 int build_sets_status(rs_ctx *c, const std::vector<const uint64_t *> &set_keys, const std::vector<uint64_t> &set_n,
-                      uint32_t leaf, uint64_t bucket, bool rotate, std::vector<rs_mphf *> &out) {
+                      uint32_t leaf, uint64_t bucket, int flags, std::vector<rs_mphf *> &out) {
     const auto t0 = std::chrono::steady_clock::now();
-    return guarded("build", c, [&] { out = build_sets(c, set_keys, set_n, leaf, bucket, rotate, false, t0); });
+    return guarded("build", c, [&] { out = build_sets(c, set_keys, set_n, leaf, bucket, flags, false, t0); });
 }
 
 uint64_t build_bound_bytes(uint64_t n, uint64_t nb) { return working_set_bound(n, nb); }
@@ -961,6 +1094,7 @@ int rs_ctx_create(int device, rs_ctx **out) {
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount;
         c->tune = parse_tune(getenv("RS_TUNE"));
+        if (const char *at = getenv("RS_AUTOTUNE")) c->autotune = atoi(at) != 0;
         HIP_CHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
         if (c->wall_khz <= 0) c->wall_khz = 100000;  // s_memrealtime: 100 MHz on gfx9 parts
         if (const char *tl = getenv("RS_TIME_LIMIT")) c->time_limit_s = std::max(0.0, atof(tl));
@@ -994,6 +1128,44 @@ int rs_ctx_set_time_limit(rs_ctx *c, double seconds) {
     }
     std::lock_guard<std::mutex> g(c->mu);
     c->time_limit_s = seconds;
+    return RS_OK;
+}
+
+int rs_autotune(rs_ctx *c, uint32_t leaf, size_t bucket, int flags, uint64_t sample_keys, char *report, size_t cap) {
+    if (!c || leaf < 2 || leaf > kMaxLeaf || bucket == 0) return set_error("rs_autotune: bad arguments"), RS_ERR_ARG;
+    std::string rep;
+    const int rc = guarded("rs_autotune", c, [&] { rep = autotune(c, leaf, bucket, flags & 3, sample_keys); });
+    if (rc) {
+        c->tuned[leaf] = false;
+        return rc;
+    }
+    if (report && cap) {
+        const size_t k = std::min(cap - 1, rep.size());
+        memcpy(report, rep.data(), k);
+        report[k] = 0;
+    }
+    return RS_OK;
+}
+
+int rs_ctx_set_tune(rs_ctx *c, uint32_t leaf, const char *tune) {
+    if (!c || !tune || leaf == 1 || leaf > kMaxLeaf) return set_error("rs_ctx_set_tune: bad arguments"), RS_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (leaf == 0) {
+        c->tune = parse_tune(tune, c->tune);
+    } else {
+        c->tune_leaf[leaf] = parse_tune(tune, c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune);
+        c->tuned[leaf] = true;
+    }
+    return RS_OK;
+}
+
+int rs_ctx_get_tune(rs_ctx *c, uint32_t leaf, char *buf, size_t cap) {
+    if (!c || !buf || !cap || leaf == 1 || leaf > kMaxLeaf) return set_error("rs_ctx_get_tune: bad arguments"), RS_ERR_ARG;
+    std::lock_guard<std::mutex> g(c->mu);
+    const std::string t = format_tune(leaf && c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune);
+    const size_t k = std::min(cap - 1, t.size());
+    memcpy(buf, t.data(), k);
+    buf[k] = 0;
     return RS_OK;
 }
 
@@ -1032,7 +1204,7 @@ int rs_build_device(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_t leaf, 
                     rs_mphf **out) {
     const auto t0 = std::chrono::steady_clock::now();
     if (int e = check_args(c, d_keys, n, leaf, bucket, out)) return e;
-    return guarded("rs_build_device", c, [&] { *out = build(c, d_keys, n, leaf, bucket, rotate != 0, t0); });
+    return guarded("rs_build_device", c, [&] { *out = build(c, d_keys, n, leaf, bucket, rotate & 3, t0); });
 }
 
 int rs_build_device_noalloc(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_t leaf, size_t bucket, int rotate,
@@ -1059,7 +1231,7 @@ int rs_build_device_noalloc(rs_ctx *c, const uint64_t *d_keys, size_t n, uint32_
             ~Reset() { t_arena = nullptr; }
         } reset;
         const SerialOut so{out, cap, out_len};
-        build_sets(c, {d_keys}, {n}, leaf, bucket, rotate != 0, false, t0, &so);
+        build_sets(c, {d_keys}, {n}, leaf, bucket, rotate & 3, false, t0, &so);
     });
 }
 
@@ -1091,7 +1263,7 @@ int rs_build_strings(rs_ctx *c, const uint8_t *bytes, const uint64_t *offsets, s
         HIP_CHECK(hipMemcpyAsync(d_off, rel.data(), (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
         k_spooky_keys<<<(n + 255) / 256, 256, 0, c->stream>>>(d_bytes, d_off, n, d_keys);
         HIP_CHECK(hipGetLastError());
-        *out = build(c, d_keys, n, leaf, bucket, rotate != 0, t0);
+        *out = build(c, d_keys, n, leaf, bucket, rotate & 3, t0);
     });
 }
 
@@ -1101,7 +1273,7 @@ int rs_build(rs_ctx *c, const uint64_t *keys, size_t n, uint32_t leaf, size_t bu
     return guarded("rs_build", c, [&] {
         uint64_t *d = c->keys_in.get<uint64_t>(2 * n);
         HIP_CHECK(hipMemcpyAsync(d, keys, n * 16, hipMemcpyHostToDevice, c->stream));
-        *out = build(c, d, n, leaf, bucket, rotate != 0, t0);
+        *out = build(c, d, n, leaf, bucket, rotate & 3, t0);
     });
 }
 
@@ -1119,7 +1291,7 @@ int rs_build_batch(rs_ctx *c, const uint64_t *d_keys, const uint64_t *set_offset
             ptrs[i] = d_keys + 2 * set_offsets[i];
             ns[i] = set_offsets[i + 1] - set_offsets[i];
         }
-        std::vector<rs_mphf *> r = build_sets(c, ptrs, ns, leaf, bucket, rotate != 0, false, t0);
+        std::vector<rs_mphf *> r = build_sets(c, ptrs, ns, leaf, bucket, rotate & 3, false, t0);
         for (size_t i = 0; i < num_sets; i++) out[i] = r[i];
         if (stats) *stats = r[0]->stats;
     });

@@ -232,6 +232,7 @@ static inline uint64_t remap128(uint64_t x, uint64_t n) { return (uint64_t)(((un
 uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     const Memo &memo = *h.memo;
     const SplitParams &P = h.P;
+    const bool r64 = (h.rotate & kFlagRemap64) != 0;
     const uint64_t bucket = remap128(first, h.nbuckets);
     uint64_t cum_keys, cum_keys_next, bit_pos;
     h.ef.get(bucket, cum_keys, cum_keys_next, bit_pos);
@@ -242,7 +243,7 @@ uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     int level = 0;
     while (m > P.upper) {
         const uint64_t d = rd.next((int)golomb_param(memo, m));
-        const uint32_t hmod = remap16(second + d + kStartSeed[level], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + kStartSeed[level], (uint32_t)m, r64);
         const uint32_t split = higher_split(P, (uint32_t)m);
         if (hmod < split) {
             m = split;
@@ -255,7 +256,7 @@ uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     }
     if (m > P.lower) {
         const uint64_t d = rd.next((int)golomb_param(memo, m));
-        const uint32_t hmod = remap16(second + d + kStartSeed[kSeedUpper], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + kStartSeed[kSeedUpper], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / P.lower;
         m = std::min<uint64_t>(P.lower, m - part * P.lower);
         cum_keys += P.lower * part;
@@ -263,7 +264,7 @@ uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     }
     if (m > P.leaf) {
         const uint64_t d = rd.next((int)golomb_param(memo, m));
-        const uint32_t hmod = remap16(second + d + kStartSeed[kSeedLower], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + kStartSeed[kSeedLower], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / P.leaf;
         m = std::min<uint64_t>(P.leaf, m - part * P.leaf);
         cum_keys += (uint64_t)P.leaf * part;
@@ -272,12 +273,12 @@ uint64_t query(const rs_mphf &h, uint64_t first, uint64_t second) {
     const uint64_t b = rd.next((int)golomb_param(memo, m));
     const uint64_t x = second + b + kStartSeed[kSeedLeaf];
     uint64_t leaf_val;
-    if (h.rotate && m == P.leaf) {
+    if ((h.rotate & kFlagRotate) && m == P.leaf) {
         const uint64_t rotation = b % P.leaf;
-        leaf_val = remap16(x - rotation, P.leaf);
+        leaf_val = remap_any(x - rotation, P.leaf, r64);
         if (second % 2 != 0) leaf_val = (leaf_val + rotation) % P.leaf;
     } else {
-        leaf_val = remap16(x, (uint32_t)m);
+        leaf_val = remap_any(x, (uint32_t)m, r64);
     }
     return cum_keys + leaf_val;
 }
@@ -397,7 +398,7 @@ int rs_mphf_deserialize(const uint8_t *buf, size_t len, uint32_t leaf_size, int 
     }
     rs_mphf *h = new rs_mphf();
     h->leaf = leaf_size;
-    h->rotate = rotate ? 1 : 0;
+    h->rotate = rotate & (kFlagRotate | kFlagRemap64);
     h->bucket_size = r.u64();
     h->keys_count = r.u64();
     r.vec(h->rice, 2);

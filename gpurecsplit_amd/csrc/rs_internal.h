@@ -60,7 +60,7 @@ struct DeviceCopy;  // device-resident query structures (rs_engine.hip)
 
 struct rs_mphf {
     uint32_t leaf = 0;
-    int rotate = 1;
+    int rotate = 1;  // flags: kFlagRotate, kFlagRemap64
     uint64_t bucket_size = 0, keys_count = 0, nbuckets = 0;
     std::vector<uint64_t> rice;  // Golomb-Rice descriptor words (with read padding beyond size)
     uint64_t rice_words = 0;     // serialized word count
@@ -78,7 +78,7 @@ void device_copy_free(DeviceCopy *d);
 // Builds one MPHF per key set on ctx (set i: set_n[i] keys at set_keys[i], device
 // memory); returns an rs_status and, on success, the handles in out.
 int build_sets_status(rs_ctx *c, const std::vector<const uint64_t *> &set_keys, const std::vector<uint64_t> &set_n,
-                      uint32_t leaf, uint64_t bucket, bool rotate, std::vector<rs_mphf *> &out);
+                      uint32_t leaf, uint64_t bucket, int flags, std::vector<rs_mphf *> &out);
 // Upper bound of the device bytes a build of n keys in nb buckets uses.
 uint64_t build_bound_bytes(uint64_t n, uint64_t nb);
 int ctx_device(const rs_ctx *c);

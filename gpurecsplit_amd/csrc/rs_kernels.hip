@@ -85,10 +85,11 @@ __device__ __forceinline__ uint32_t h16_of(uint32_t t) {
 }
 __device__ __forceinline__ uint32_t search_h16(uint64_t z) { return h16_of(hash_pre(z)); }
 
-// LDS byte address of entry (h16 * m) >> 16 of a word table that starts at word
-// `base_hi >> 16`: one 24-bit multiply-add, then the upper half of the product
-// shifted left by 2 in one word-select shift.  h16 * m < 2^28 and the table base
-// is below 2^12 words, so the sum fits 32 bits and its upper half is the index.
+// LDS byte address of entry (h16 * m) >> 16 of a table of 2^shift-byte entries
+// that starts at entry `base_hi >> 16`: one 24-bit multiply-add, then the upper
+// half of the sum shifted left by `shift` in one word-select shift.  h16 * m <
+// 2^28 and the table base is below 2^15 entries, so the sum fits 32 bits and its
+// upper half is the entry index.
 __device__ __forceinline__ uint32_t lut_byte_addr(uint32_t h16, uint32_t m, uint32_t base_hi, uint32_t two) {
     const uint32_t p = __umul24(h16, m) + base_hi;
     uint32_t a;
@@ -443,15 +444,31 @@ __device__ __forceinline__ void each_hash(const uint64_t *__restrict__ kp, uint3
     }
 }
 
+// 64-bit-remap form of each_hash: calls g(u, r) with r = remap64(key + base +
+// off[u], m) (rs_common.h), keys in scalar registers.
+template <int U, typename G>
+__device__ __forceinline__ void each_r64(const uint64_t *__restrict__ kp, uint32_t begin, uint32_t end,
+                                         const Seeds<U> &sd, uint32_t m, G &&g) {
+    const uint64_t base = uni64(sd.base);
+    stream_keys_scalar(kp, begin, end, [&](uint64_t k) {
+        const uint64_t S = k + base;
+#pragma unroll
+        for (int u = 0; u < U; u++) g(u, remap64(S + sd.off[u], m));
+    });
+}
+
 // Fanout-2 node: lowest x with #{remap(k+S[depth]+x, m) < split} == split
 // (SIMDRecSplit.hpp:250-317).  remap < split <=> h16 < ceil(split*2^16/m).
-template <int U, bool SK>
+template <int U, bool SK, bool R64>
 __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                  uint32_t split, uint32_t H, uint64_t u0) {
     uint32_t cnt[U];
 #pragma unroll
     for (int u = 0; u < U; u++) cnt[u] = 0;
-    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += h < H; });
+    if constexpr (R64)
+        each_r64<U>(kp, 0, m, Seeds<U>(u0 + sconst, 1), m, [&](int u, uint32_t r) { cnt[u] += r < split; });
+    else
+        each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += h < H; });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(cnt[u] == split);
@@ -466,7 +483,7 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
 // forces another count negative, so equality of the counted fields is
 // equality of all counts.  With LUT, the increment 1 << (fb * part) of every
 // remapped value r < m is read from a per-wave LDS table instead of computed.
-template <int U, bool WIDE, bool LUT, bool SK>
+template <int U, bool WIDE, bool LUT, bool SK, bool R64>
 __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask,
                                                uint32_t lut_hi, uint32_t two, uint64_t u0) {
@@ -474,14 +491,28 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
     C cnt[U];
 #pragma unroll
     for (int u = 0; u < U; u++) cnt[u] = 0;
-    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) {
-        if (LUT) {
-            cnt[u] += *(const __attribute__((address_space(3))) uint32_t *)(uintptr_t)lut_byte_addr(h, m, lut_hi, two);
-        } else {
-            const uint32_t part = __umul24(__umul24(h, m) >> 16, magic) >> 20;
-            cnt[u] += (C)1 << __umul24(part, fb);
-        }
-    });
+    using LutT = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    if constexpr (R64) {
+        // LDS table at entry lut_hi >> 16, indexed by the remapped value
+        const uint32_t lut_b = (lut_hi >> 16) * (uint32_t)sizeof(LutT);
+        each_r64<U>(kp, 0, m, Seeds<U>(u0 + sconst, 1), m, [&](int u, uint32_t r) {
+            if (LUT) {
+                cnt[u] += *(const __attribute__((address_space(3))) LutT *)(uintptr_t)(lut_b + (uint32_t)sizeof(LutT) * r);
+            } else {
+                const uint32_t part = __umul24(r, magic) >> 20;
+                cnt[u] += (C)1 << __umul24(part, fb);
+            }
+        });
+    } else {
+        each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) {
+            if (LUT) {
+                cnt[u] += *(const __attribute__((address_space(3))) LutT *)(uintptr_t)lut_byte_addr(h, m, lut_hi, two);
+            } else {
+                const uint32_t part = __umul24(__umul24(h, m) >> 16, magic) >> 20;
+                cnt[u] += (C)1 << __umul24(part, fb);
+            }
+        });
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(((uint64_t)cnt[u] & cmask) == T);
@@ -494,7 +525,7 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
 // The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
-template <int U, bool SK>
+template <int U, bool SK, bool R64>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
                                                    uint32_t nr, uint64_t u0) {
     const uint64_t S = c_start_seed[kSeedLeaf];
@@ -503,8 +534,13 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
 #pragma unroll
     for (int u = 0; u < U; u++) ml[u] = mr[u] = 0;
     const Seeds<U> sd(u0 * L + S, L);
-    each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
-    each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
+    if constexpr (R64) {
+        each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { ml[u] |= 1u << r; });
+        each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mr[u] |= 1u << r; });
+    } else {
+        each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
+        each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
+    }
     // Candidates whose two masks are collision-free are tested in index order, one
     // at a time across the wave: lane r tests rotation r, so one step covers all L
     // rotations and the lowest fitting rotation is the lowest set bit of a ballot.
@@ -528,7 +564,7 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
 // Leaf without rotation (SIMDRecSplit.hpp:510-599): lowest x whose m keys land on
 // m distinct positions.  A prefix of ceil(3 sqrt(m)) keys must already be
 // collision-free; waves skip the rest when no lane passes (lossless).
-template <int U, bool SK>
+template <int U, bool SK, bool R64>
 __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict__ kp, uint32_t m, uint32_t mid,
                                                      uint64_t u0) {
     const uint64_t S = c_start_seed[kSeedLeaf];
@@ -537,13 +573,17 @@ __device__ __forceinline__ uint64_t group_leaf_brute(const uint64_t *__restrict_
 #pragma unroll
     for (int u = 0; u < U; u++) mk[u] = 0;
     const Seeds<U> sd(u0 + S, 1);
-    each_hash<U, SK>(kp, 0, mid, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, m) >> 16); });
+    auto part = [&](uint32_t b, uint32_t e) {
+        if constexpr (R64) each_r64<U>(kp, b, e, sd, m, [&](int u, uint32_t r) { mk[u] |= 1u << r; });
+        else each_hash<U, SK>(kp, b, e, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, m) >> 16); });
+    };
+    part(0, mid);
     if (mid < m) {
         bool any = false;
 #pragma unroll
         for (int u = 0; u < U; u++) any |= (uint32_t)__popc(mk[u]) == mid;
         if (!__ballot(any)) return kNone;
-        each_hash<U, SK>(kp, mid, m, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, m) >> 16); });
+        part(mid, m);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -559,18 +599,22 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-template <int KIND, int U, bool WIDE, bool LUT, bool SK>
+template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64>
 __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
+    // Aggregation increment table of this wave: 4-byte entries, or 8-byte entries
+    // for 64-bit counters (WIDE); lut_stride counts 4-byte words and is even.
+    constexpr uint32_t kLutShift = WIDE ? 3 : 2;
     uint32_t *lut = s_lut + (threadIdx.x >> 6) * a.lut_stride;
-    // table start in LDS words, in the upper half (the low 32 bits of a flat LDS
+    // table start in entries, in the upper half (the low 32 bits of a flat LDS
     // address are its LDS offset)
-    const uint32_t lut_hi = uni((((uint32_t)(uintptr_t)s_lut >> 2) + (threadIdx.x >> 6) * a.lut_stride) << 16);
-    const uint32_t two = 2;
+    const uint32_t lut_hi = uni(
+        (((uint32_t)(uintptr_t)s_lut >> kLutShift) + (threadIdx.x >> 6) * (a.lut_stride >> (kLutShift - 2))) << 16);
+    const uint32_t two = kLutShift;
     int node = -1;
     bool helping = false;
     uint32_t scan = wave;
@@ -640,7 +684,11 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 const uint32_t w = (fan - 1) * a.field_bits;
                 cmask = w >= 64 ? ~0ull : ((1ull << w) - 1);
                 if (LUT) {
-                    for (uint32_t r = lane; r < m; r += 64) lut[r] = 1u << (a.field_bits * (r / su));
+                    if (WIDE)
+                        for (uint32_t r = lane; r < m; r += 64)
+                            ((uint64_t *)lut)[r] = 1ull << (a.field_bits * (r / su));
+                    else
+                        for (uint32_t r = lane; r < m; r += 64) lut[r] = 1u << (a.field_bits * (r / su));
                     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
                 }
             } else if (KIND == kLeafRot) {
@@ -701,11 +749,12 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 if (u0 * scale >= r) break;
             }
             work += (uint64_t)U * 64 * m;
-            if (KIND == kHigher) hit = group_higher<U, SK>(kp, m, sconst, aux0, aux1, u0);
+            if (KIND == kHigher) hit = group_higher<U, SK, R64>(kp, m, sconst, aux0, aux1, u0);
             else if (KIND == kUpper || KIND == kLower)
-                hit = group_aggr<U, WIDE, LUT, SK>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two, u0);
-            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK>(kp, m, aux0, aux1, u0);
-            else hit = group_leaf_brute<U, SK>(kp, m, aux0, u0);
+                hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
+                                                        u0);
+            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64>(kp, m, aux0, aux1, u0);
+            else hit = group_leaf_brute<U, SK, R64>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
         if (hit != kNone && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)hit);
@@ -742,7 +791,7 @@ __global__ __launch_bounds__(256) void k_finalize(const NodeDesc *__restrict__ n
 // so this layout does not change any seed.
 __global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
                                               const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
-                                              uint32_t *__restrict__ err, SplitParams P, uint32_t kind) {
+                                              uint32_t *__restrict__ err, SplitParams P, uint32_t kind, int r64) {
     extern __shared__ uint64_t s_keys[];
     const uint32_t node = blockIdx.x;
     const uint32_t lane = threadIdx.x;
@@ -769,7 +818,7 @@ __global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const
         fan = (m + unit - 1) / unit;
     }
     auto child_of = [&](uint64_t key) -> uint32_t {
-        const uint32_t r = remap16(key + seed_abs, m);
+        const uint32_t r = remap_any(key + seed_abs, m, r64 != 0);
         return kind == kHigher ? (r >= unit ? 1u : 0u) : r / unit;
     };
     // pass 1: even keys per child
@@ -1082,7 +1131,7 @@ struct QueryArgs {
     uint64_t n_rice, n_lower, n_uc, n_up, n_jump;
     uint64_t keys_count;
     SplitParams P;
-    int rotate;
+    int rotate;  // flags: kFlagRotate, kFlagRemap64
 };
 
 __device__ __forceinline__ uint64_t rd_word(const uint64_t *base, uint64_t n, uint64_t i) {
@@ -1179,10 +1228,11 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
         fixed_off += flen;
     };
     const SplitParams P = q.P;
+    const bool r64 = (q.rotate & kFlagRemap64) != 0;
     int level = 0;
     while (m > P.upper) {
         const uint64_t d = read_next(q.memo[m] >> 27);
-        const uint32_t hmod = remap16(second + d + c_start_seed[level], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + c_start_seed[level], (uint32_t)m, r64);
         const uint32_t split = higher_split(P, (uint32_t)m);
         if (hmod < split) {
             m = split;
@@ -1195,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
     }
     if (m > P.lower) {
         const uint64_t d = read_next(q.memo[m] >> 27);
-        const uint32_t hmod = remap16(second + d + c_start_seed[kSeedUpper], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + c_start_seed[kSeedUpper], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / P.lower;
         m = min((uint64_t)P.lower, m - part * P.lower);
         cum_keys += P.lower * part;
@@ -1203,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
     }
     if (m > P.leaf) {
         const uint64_t d = read_next(q.memo[m] >> 27);
-        const uint32_t hmod = remap16(second + d + c_start_seed[kSeedLower], (uint32_t)m);
+        const uint32_t hmod = remap_any(second + d + c_start_seed[kSeedLower], (uint32_t)m, r64);
         const uint64_t part = (uint16_t)hmod / P.leaf;
         m = min((uint64_t)P.leaf, m - part * P.leaf);
         cum_keys += (uint64_t)P.leaf * part;
@@ -1212,12 +1262,12 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs q, const uint64_t *__re
     const uint64_t b = read_next(q.memo[m] >> 27);
     const uint64_t x = second + b + c_start_seed[kSeedLeaf];
     uint64_t v;
-    if (q.rotate && m == P.leaf) {
+    if ((q.rotate & kFlagRotate) && m == P.leaf) {
         const uint64_t rot = b % P.leaf;
-        v = remap16(x - rot, P.leaf);
+        v = remap_any(x - rot, P.leaf, r64);
         if (second & 1) v = (v + rot) % P.leaf;
     } else {
-        v = remap16(x, (uint32_t)m);
+        v = remap_any(x, (uint32_t)m, r64);
     }
     out[t] = cum_keys + v;
 }

@@ -126,7 +126,7 @@ struct Job {
     uint64_t id = 0;
     uint32_t leaf = 0;
     uint64_t bucket = 0;
-    int rotate = 1;
+    int rotate = 1;  // flags
     uint64_t n = 0;
     std::vector<uint64_t> host_keys;  // only while waiting for key memory
     const uint64_t *d_keys = nullptr;
@@ -280,7 +280,7 @@ struct rs_server {
                 ns.push_back(j->n);
             }
             std::vector<rs_mphf *> out;
-            int rc = build_sets_status(slot->ctx, ptrs, ns, batch[0]->leaf, batch[0]->bucket, batch[0]->rotate != 0, out);
+            int rc = build_sets_status(slot->ctx, ptrs, ns, batch[0]->leaf, batch[0]->bucket, batch[0]->rotate, out);
             std::vector<int> codes(batch.size(), rc);
             std::vector<std::string> errs(batch.size(), rc ? rs_last_error() : "");
             out.resize(batch.size(), nullptr);
@@ -289,7 +289,7 @@ struct rs_server {
                 for (size_t i = 0; i < batch.size(); i++) {
                     std::vector<rs_mphf *> one;
                     codes[i] = build_sets_status(slot->ctx, {ptrs[i]}, {ns[i]}, batch[i]->leaf, batch[i]->bucket,
-                                                 batch[i]->rotate != 0, one);
+                                                 batch[i]->rotate, one);
                     errs[i] = codes[i] ? rs_last_error() : "";
                     out[i] = codes[i] == RS_OK ? one[0] : nullptr;
                 }
@@ -397,7 +397,7 @@ static int submit(rs_server *s, const uint64_t *keys, bool device_keys, size_t n
     auto j = std::make_shared<Job>();
     j->leaf = leaf;
     j->bucket = bucket;
-    j->rotate = rotate ? 1 : 0;
+    j->rotate = rotate & (kFlagRotate | kFlagRemap64);
     j->n = n;
     std::unique_lock<std::mutex> lk(s->mu);
     if (s->closing) return set_error("rs_submit_job: server is closing"), RS_ERR_ARG;

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import GPURecSplit, RecSplitError, _check, _keys_array, lib
+from . import GPURecSplit, RecSplitError, _check, _flags, _keys_array, _live, lib
 
 __all__ = ["Server", "Job", "poisson_stream", "run_stream"]
 
@@ -81,8 +81,9 @@ class Server:
         _check(lib.rs_server_create(C.byref(cfg), C.byref(h)))
         self.h = h
         self.device = device
+        _live["server"].add(self)
 
-    def submit(self, keys, leaf: int, bucket: int, rotate: bool = True) -> int:
+    def submit(self, keys, leaf: int, bucket: int, rotate: bool = True, remap64: bool = False) -> int:
         """Submits one job; keys: (n, 2) uint64 host array (copied at once) or a
         contiguous (n, 2) 64-bit CUDA tensor (used in place: keep it alive until wait)."""
         jid = C.c_uint64()
@@ -90,11 +91,11 @@ class Server:
             import torch
             torch.cuda.current_stream(keys.device).synchronize()
             _check(lib.rs_submit_job_device(self.h, C.c_void_p(keys.data_ptr()), keys.shape[0], leaf, bucket,
-                                            int(rotate), C.byref(jid)))
+                                            _flags(rotate, remap64), C.byref(jid)))
         else:
             k = _keys_array(keys)
-            _check(lib.rs_submit_job(self.h, k.ctypes.data_as(_u64p), k.shape[0], leaf, bucket, int(rotate),
-                                     C.byref(jid)))
+            _check(lib.rs_submit_job(self.h, k.ctypes.data_as(_u64p), k.shape[0], leaf, bucket,
+                                     _flags(rotate, remap64), C.byref(jid)))
         return int(jid.value)
 
     def poll(self, job_id: int) -> dict:

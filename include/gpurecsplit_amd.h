@@ -137,6 +137,21 @@ int rs_build_device_noalloc(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint3
 int rs_build_batch(rs_ctx *ctx, const uint64_t *d_keys, const uint64_t *set_offsets, size_t num_sets,
                    uint32_t leaf_size, size_t bucket_size, int rotate, rs_mphf **out, rs_build_stats *stats);
 
+/* Launch tuning of the seed search (no reference counterpart; the reference fixes
+ * its launch constants, GPURecSplit.cuh:420-425).  rs_autotune times builds of a
+ * synthetic sample (sample_keys SplitMix64 keys; 0 sizes it to ~40 ms per build)
+ * for the leaf size, bucket size and flags, picks per node kind the candidates per
+ * lane and per claimed chunk, then the workgroups per CU, and makes the result the
+ * context's tuning for that leaf size; report (may be NULL) receives a JSON
+ * summary.  RS_AUTOTUNE=1 in the environment tunes each leaf size at its first
+ * build on a context.  A tuning is a string "u2=4,g2=2,bpc=8,..." (the RS_TUNE
+ * syntax): rs_ctx_set_tune applies one to a leaf size (0: the default for all
+ * leaf sizes), rs_ctx_get_tune returns the one in effect. */
+int rs_autotune(rs_ctx *ctx, uint32_t leaf_size, size_t bucket_size, int rotate, uint64_t sample_keys, char *report,
+                size_t cap);
+int rs_ctx_set_tune(rs_ctx *ctx, uint32_t leaf_size, const char *tune);
+int rs_ctx_get_tune(rs_ctx *ctx, uint32_t leaf_size, char *buf, size_t cap);
+
 /* Device memory the context holds for reuse by later builds (grow-only buffers),
  * and their release.  No reference counterpart. */
 uint64_t rs_ctx_device_bytes(rs_ctx *ctx);

@@ -4,6 +4,7 @@
 // Reference surface (paths relative to the ByteHamster/GpuRecSplit root):
 //   template <size_t LEAF_SIZE, sux::util::AllocType AT, bool USE_BIJECTIONS_ROTATE>
 //   class GPURecSplit                         include/function/GPURecSplit.cuh:409-411
+//   class recsplit_rotate::RecSplit           include/function/RecSplitRotate.hpp:115 (64-bit remap)
 //   GPURecSplit()                             :432
 //   GPURecSplit(vector<hash128_t>&, size_t bucket_size, int numThreads)   :484-489
 //   size_t operator()(const hash128_t&)       include/function/AbstractParallelRecSplit.hpp:409-466
@@ -91,32 +92,39 @@ typedef struct __hash128_t {
 static_assert(sizeof(hash128_t) == 16, "hash128_t must be two packed uint64");
 #endif
 
-template <size_t LEAF_SIZE, sux::util::AllocType AT = sux::util::AllocType::MALLOC, bool USE_BIJECTIONS_ROTATE = true>
-class GPURecSplit {
+namespace detail {
+
+// The construction engine behind both reference classes: GPURecSplit (16-bit remap)
+// and recsplit_rotate::RecSplit (64-bit remap); REMAP64 selects the hash-to-child
+// mapping (AbstractParallelRecSplit.hpp:327-337).
+template <size_t LEAF_SIZE, sux::util::AllocType AT, bool USE_BIJECTIONS_ROTATE, bool REMAP64>
+class RecSplitImpl {
     static_assert(LEAF_SIZE >= 2 && LEAF_SIZE <= 24, "leaf size must be in [2, 24]");
+    // the C ABI's flags word: 1 rotation fitting, 2 the 64-bit remap
+    static constexpr int kFlags = (USE_BIJECTIONS_ROTATE ? 1 : 0) | (REMAP64 ? 2 : 0);
     rs_mphf *h = nullptr;
 
   public:
-    GPURecSplit() {}
+    RecSplitImpl() {}
 
     /** Builds the MPHF of `keys` (distinct 128-bit hashes) with the given bucket size. */
-    GPURecSplit(std::vector<hash128_t> &keys, const size_t bucket_size, const int numThreads = 1) {
+    RecSplitImpl(std::vector<hash128_t> &keys, const size_t bucket_size, const int numThreads = 1) {
         (void)numThreads;
         gpurecsplit_amd::check(rs_build(gpurecsplit_amd::default_context(),
                                         reinterpret_cast<const uint64_t *>(keys.data()), keys.size(), LEAF_SIZE,
-                                        bucket_size, USE_BIJECTIONS_ROTATE ? 1 : 0, &h),
+                                        bucket_size, kFlags, &h),
                                "GPURecSplit");
     }
 
     /** Builds the MPHF of distinct string keys (GPURecSplit.cuh:443-478): every key is
      *  replaced by its 128-bit SpookyHash V2 first_hash, computed on the GPU. */
-    GPURecSplit(const std::vector<std::string> &keys, const size_t bucket_size, const int numThreads = 1) {
+    RecSplitImpl(const std::vector<std::string> &keys, const size_t bucket_size, const int numThreads = 1) {
         (void)numThreads;
         build_strings(keys, bucket_size);
     }
 
     /** Builds from the keys of a stream, one per line (GPURecSplit.cuh:498-505). */
-    GPURecSplit(std::ifstream &input, const size_t bucket_size, const int numThreads = 1) {
+    RecSplitImpl(std::ifstream &input, const size_t bucket_size, const int numThreads = 1) {
         (void)numThreads;
         std::vector<std::string> keys;
         for (std::string key; std::getline(input, key);) keys.push_back(std::move(key));
@@ -124,19 +132,19 @@ class GPURecSplit {
     }
 
     /** Builds from keys already resident in device memory of the context's GPU. */
-    static GPURecSplit fromDevice(const hash128_t *d_keys, size_t n, size_t bucket_size) {
-        GPURecSplit r;
+    static RecSplitImpl fromDevice(const hash128_t *d_keys, size_t n, size_t bucket_size) {
+        RecSplitImpl r;
         gpurecsplit_amd::check(rs_build_device(gpurecsplit_amd::default_context(),
                                                reinterpret_cast<const uint64_t *>(d_keys), n, LEAF_SIZE, bucket_size,
-                                               USE_BIJECTIONS_ROTATE ? 1 : 0, &r.h),
+                                               kFlags, &r.h),
                                "GPURecSplit::fromDevice");
         return r;
     }
 
-    GPURecSplit(const GPURecSplit &) = delete;
-    GPURecSplit &operator=(const GPURecSplit &) = delete;
-    GPURecSplit(GPURecSplit &&o) noexcept : h(o.h) { o.h = nullptr; }
-    GPURecSplit &operator=(GPURecSplit &&o) noexcept {
+    RecSplitImpl(const RecSplitImpl &) = delete;
+    RecSplitImpl &operator=(const RecSplitImpl &) = delete;
+    RecSplitImpl(RecSplitImpl &&o) noexcept : h(o.h) { o.h = nullptr; }
+    RecSplitImpl &operator=(RecSplitImpl &&o) noexcept {
         if (this != &o) {
             rs_mphf_free(h);
             h = o.h;
@@ -144,7 +152,7 @@ class GPURecSplit {
         }
         return *this;
     }
-    ~GPURecSplit() { rs_mphf_free(h); }
+    ~RecSplitImpl() { rs_mphf_free(h); }
 
     /** Returns the value in [0, n) associated with the given 128-bit hash. */
     size_t operator()(const hash128_t &hash) { return (size_t)rs_mphf_query(h, hash.first, hash.second); }
@@ -181,13 +189,13 @@ class GPURecSplit {
         for (size_t i = 0; i < keys.size(); i++)
             if (!keys[i].empty()) std::memcpy(bytes.data() + offsets[i], keys[i].data(), keys[i].size());
         gpurecsplit_amd::check(rs_build_strings(gpurecsplit_amd::default_context(), bytes.data(), offsets.data(),
-                                                keys.size(), LEAF_SIZE, bucket_size, USE_BIJECTIONS_ROTATE ? 1 : 0, &h),
+                                                keys.size(), LEAF_SIZE, bucket_size, kFlags, &h),
                                "GPURecSplit(strings)");
     }
 
   public:
 
-    friend std::ostream &operator<<(std::ostream &os, const GPURecSplit &rs) {
+    friend std::ostream &operator<<(std::ostream &os, const RecSplitImpl &rs) {
         std::vector<uint8_t> buf(rs_mphf_serialized_size(rs.h));
         gpurecsplit_amd::check(rs_mphf_serialize(rs.h, buf.data(), buf.size()), "operator<<");
         os.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)buf.size());
@@ -198,7 +206,7 @@ class GPURecSplit {
      *  header words, the descriptor vector, the six Elias-Fano scalars and its four
      *  vectors, each vector as a word count followed by the words (DoubleEF.hpp:115-135).
      *  Bytes after the structure stay in the stream. */
-    friend std::istream &operator>>(std::istream &is, GPURecSplit &rs) {
+    friend std::istream &operator>>(std::istream &is, RecSplitImpl &rs) {
         std::vector<uint8_t> buf;
         auto take = [&](size_t bytes) {
             const size_t at = buf.size();
@@ -225,12 +233,26 @@ class GPURecSplit {
         for (int i = 0; i < 6; i++) word();
         for (int i = 0; i < 4; i++) vec();
         rs_mphf *nh = nullptr;
-        gpurecsplit_amd::check(rs_mphf_deserialize(buf.data(), buf.size(), LEAF_SIZE, USE_BIJECTIONS_ROTATE ? 1 : 0, &nh),
+        gpurecsplit_amd::check(rs_mphf_deserialize(buf.data(), buf.size(), LEAF_SIZE, kFlags, &nh),
                                "operator>>");
         rs_mphf_free(rs.h);
         rs.h = nh;
         return is;
     }
 };
+
+}  // namespace detail
+
+/** The reference's GPURecSplit<LEAF_SIZE, AT, USE_BIJECTIONS_ROTATE> (GPURecSplit.cuh:409-411). */
+template <size_t LEAF_SIZE, sux::util::AllocType AT = sux::util::AllocType::MALLOC, bool USE_BIJECTIONS_ROTATE = true>
+using GPURecSplit = detail::RecSplitImpl<LEAF_SIZE, AT, USE_BIJECTIONS_ROTATE, false>;
+
+namespace recsplit_rotate {
+/** The reference's recsplit_rotate::RecSplit<LEAF_SIZE, AT, USE_BIJECTIONS_ROTATE>
+ *  (RecSplitRotate.hpp:115): the same construction with the 64-bit remap; built on
+ *  the GPU here (the reference builds it on host threads). */
+template <size_t LEAF_SIZE, sux::util::AllocType AT = sux::util::AllocType::MALLOC, bool USE_BIJECTIONS_ROTATE = true>
+using RecSplit = detail::RecSplitImpl<LEAF_SIZE, AT, USE_BIJECTIONS_ROTATE, true>;
+}  // namespace recsplit_rotate
 
 }  // namespace bez::function

@@ -8,6 +8,7 @@
 //   dropin_test buildstr <keys.txt> <leaf> <bucket> <out.bin>
 //                                                   string constructor from an ifstream (GPU)
 //   dropin_test build <keys.bin> <leaf> <bucket> <out.bin>
+//   dropin_test load64 / build64 ...  the same with recsplit_rotate::RecSplit (64-bit remap)
 //                                                   constructor + operator() + operator<< (GPU)
 // keys.bin holds n little-endian (first, second) uint64 pairs.
 #include <gpurecsplit_amd/GPURecSplit.hpp>
@@ -20,6 +21,7 @@
 
 using bez::function::GPURecSplit;
 using bez::function::hash128_t;
+namespace rr = bez::function::recsplit_rotate;
 
 static std::vector<hash128_t> read_keys(const char *path) {
     std::ifstream f(path, std::ios::binary);
@@ -45,11 +47,13 @@ static int check_bijection(RS &rs, const std::vector<hash128_t> &keys) {
     return 0;
 }
 
-template <size_t L>
+template <size_t L, template <size_t, sux::util::AllocType, bool> class RS = GPURecSplit>
 static int run(int argc, char **argv) {
-    const std::string mode = argv[1];
+    using GPURecSplit = RS<L, sux::util::AllocType::MALLOC, true>;
+    std::string mode = argv[1];
+    if (mode.size() > 2 && mode.compare(mode.size() - 2, 2, "64") == 0) mode = mode.substr(0, mode.size() - 2);
     if (mode == "load") {
-        GPURecSplit<L> rs;
+        GPURecSplit rs;
         std::ifstream in(argv[2], std::ios::binary);
         in >> rs;
         auto keys = read_keys(argv[3]);
@@ -63,13 +67,13 @@ static int run(int argc, char **argv) {
             std::ifstream f(mode == "loadstr" ? argv[3] : argv[2]);
             for (std::string k; std::getline(f, k);) keys.push_back(k);
         }
-        GPURecSplit<L> rs;
+        GPURecSplit rs;
         if (mode == "loadstr") {
             std::ifstream in(argv[2], std::ios::binary);
             in >> rs;
         } else {
             std::ifstream in(argv[2]);
-            rs = GPURecSplit<L>(in, std::stoul(argv[4]));
+            rs = GPURecSplit(in, std::stoul(argv[4]));
             std::ofstream out(argv[5], std::ios::binary);
             out << rs;
         }
@@ -86,7 +90,7 @@ static int run(int argc, char **argv) {
         return 0;
     }
     if (mode == "load2") {
-        GPURecSplit<L> a, b;
+        GPURecSplit a, b;
         std::ifstream in(argv[2], std::ios::binary);
         in >> a >> b;
         std::string rest((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
@@ -97,7 +101,7 @@ static int run(int argc, char **argv) {
     }
     auto keys = read_keys(argv[2]);
     const auto before = keys;
-    GPURecSplit<L> rs(keys, std::stoul(argv[4]));
+    GPURecSplit rs(keys, std::stoul(argv[4]));
     if (std::memcmp(before.data(), keys.data(), keys.size() * 16) != 0) {
         std::printf("caller keys were modified\n");
         return 1;
@@ -115,13 +119,15 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "usage: see source\n");
         return 2;
     }
-    const std::string mode = argv[1];
+    std::string mode = argv[1];
+    const bool r64 = mode.size() > 2 && mode.compare(mode.size() - 2, 2, "64") == 0;
+    if (r64) mode = mode.substr(0, mode.size() - 2);
     const int leaf = std::stoi(mode == "load" || mode == "loadstr" ? argv[4] : mode == "load2" ? argv[5] : argv[3]);
     try {
         switch (leaf) {
-        case 8: return run<8>(argc, argv);
-        case 12: return run<12>(argc, argv);
-        case 16: return run<16>(argc, argv);
+        case 8: return r64 ? run<8, rr::RecSplit>(argc, argv) : run<8>(argc, argv);
+        case 12: return r64 ? run<12, rr::RecSplit>(argc, argv) : run<12>(argc, argv);
+        case 16: return r64 ? run<16, rr::RecSplit>(argc, argv) : run<16>(argc, argv);
         default: std::fprintf(stderr, "leaf size %d not compiled in\n", leaf); return 2;
         }
     } catch (const gpurecsplit_amd::Error &e) {

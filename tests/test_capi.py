@@ -150,3 +150,30 @@ def test_bits_count_matches_reference_formula():
 def test_time_limit_rejects_null_context():
     assert gra.lib.rs_ctx_set_time_limit(None, 1.0) == 1
     assert gra.lib.rs_ctx_set_time_limit(None, -1.0) == 1
+
+
+@pytest.mark.parametrize("leaf,bucket,n,flags", [(8, 100, 20000, 3), (16, 2000, 4000, 3), (5, 7, 3000, 2), (2, 7, 500, 3)])
+def test_host_query_remap64_on_oracle_bytes(leaf, bucket, n, flags):
+    """The RecSplitRotate variant (64-bit remap): a structure serialized by the
+    oracle loads with remap64 and answers every query as the oracle does."""
+    keys = splitmix_keys(2000 + leaf, n)
+    ref = OracleMPHF.build(keys, leaf, bucket, rotate=flags)
+    data = ref.serialize()
+    rs = gra.GPURecSplit.deserialize(data, leaf, rotate=bool(flags & 1), remap64=True)
+    assert rs.serialize() == data
+    got = rs.query(keys)
+    assert np.array_equal(got, ref.query(keys))
+    assert np.array_equal(np.sort(got), np.arange(n, dtype=np.uint64))
+    # the 16-bit reading of the same bytes is a different function
+    assert not np.array_equal(gra.GPURecSplit.deserialize(data, leaf, rotate=bool(flags & 1)).query(keys), got)
+
+
+def test_tune_strings_without_gpu():
+    """Tuning strings are C ABI data: rs_ctx_set_tune / rs_ctx_get_tune need a
+    context (a GPU); without one, context creation fails cleanly."""
+    h = C.c_void_p()
+    rc = gra.lib.rs_ctx_create(0, C.byref(h))
+    if rc == 0:
+        gra.lib.rs_ctx_destroy(h)
+        pytest.skip("a GPU is visible")
+    assert rc == 2 and b"device" in gra.lib.rs_last_error()

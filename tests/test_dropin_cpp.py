@@ -48,12 +48,33 @@ def test_cpp_load_rejects_truncated_stream(tmp_path):
     assert r.returncode == 3 and "error 8" in r.stdout
 
 
+def test_cpp_recsplit_rotate_load_and_query(tmp_path):
+    """recsplit_rotate::RecSplit (64-bit remap) reads the oracle's bytes and queries
+    a bijection."""
+    keys = splitmix_keys(64, 20000)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    (tmp_path / "mphf.bin").write_bytes(OracleMPHF.build(keys, 16, 2000, rotate=3).serialize())
+    r = _run("load64", tmp_path / "mphf.bin", tmp_path / "keys.bin", 16)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "load ok" in r.stdout
+
+
 def test_cpp_load_rejects_leaf_mismatch(tmp_path):
     keys = splitmix_keys(5, 1000)
     (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
     (tmp_path / "mphf.bin").write_bytes(OracleMPHF.build(keys, 8, 100).serialize())
     r = _run("load", tmp_path / "mphf.bin", tmp_path / "keys.bin", 12)
     assert r.returncode == 3 and "error 8" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("leaf,bucket,n", [(8, 100, 50000), (12, 100, 20000)])
+def test_cpp_recsplit_rotate_build_matches_oracle(tmp_path, leaf, bucket, n):
+    keys = splitmix_keys(99 + leaf, n)
+    (tmp_path / "keys.bin").write_bytes(keys.astype("<u8").tobytes())
+    r = _run("build64", tmp_path / "keys.bin", leaf, bucket, tmp_path / "out.bin")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "out.bin").read_bytes() == OracleMPHF.build(keys, leaf, bucket, rotate=3, threads=8).serialize()
 
 
 @pytest.mark.gpu

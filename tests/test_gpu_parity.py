@@ -232,3 +232,22 @@ def test_wide_counters_use_high_word(torch_cuda):
     64-bit counters carry fields above bit 31."""
     keys = splitmix_keys(20, 900)
     _parity(keys, 20, 900, threads=1)
+
+
+R64_CASES = [(2, 7, 2000, True), (5, 10, 5000, False), (8, 100, 50000, True), (8, 100, 20000, False),
+             (12, 100, 30000, True), (16, 2000, 10000, True), (18, 50, 3000, True), (24, 10, 100, True)]
+
+
+@pytest.mark.parametrize("leaf,bucket,n,rotate", R64_CASES)
+def test_remap64_bytes_match_oracle(torch_cuda, leaf, bucket, n, rotate):
+    """The RecSplitRotate variant (64-bit remap, RecSplitRotate.hpp:115): GPU bytes
+    equal the oracle's, and the GPU query kernel agrees with the oracle's query."""
+    torch = torch_cuda
+    keys = splitmix_keys(leaf * 31 + n, n)
+    flags = (1 if rotate else 0) | 2
+    _log(f"remap64 leaf={leaf} bucket={bucket} n={n} rotate={rotate}")
+    ref = OracleMPHF.build(keys, leaf, bucket, rotate=flags, threads=8)
+    rs = gra.GPURecSplit(keys, bucket, leaf_size=leaf, rotate=rotate, remap64=True)
+    assert rs.serialize() == ref.serialize()
+    d = torch.from_numpy(keys.view(np.int64)).cuda()
+    assert np.array_equal(rs.query_device(d).cpu().numpy().view(np.uint64), ref.query(keys))
