@@ -46,7 +46,7 @@ KINDS = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforc
 # HBM bytes per launch of the search kernels at the plain l = 16 workload, from
 # two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0: the factors of
 # a 256 MiB copy probe); the profile summaries under profiles/ give the commands.
-TRAFFIC_BYTES = {"lower_aggr": 8.095e9, "upper_aggr": 2.083e9, "leaf_rotation": 4.820e9}
+TRAFFIC_BYTES = {"lower_aggr": 15.55e9, "upper_aggr": 4.73e9, "leaf_rotation": 4.63e9}
 
 
 def log(msg: str):

@@ -34,6 +34,10 @@ constexpr int kSeedUpper = 17, kSeedLower = 18, kSeedLeaf = 19;
 // Node kinds; also the index into rs_build_stats::search_ms / evals / nodes.
 enum NodeKind : uint32_t { kHigher = 0, kUpper = 1, kLower = 2, kLeafRot = 3, kLeafBrute = 4, kNumKinds = 5 };
 
+// Phase lists of a build: fanout-2 nodes by depth, then upper aggregation, lower
+// aggregation, rotation leaves, brute-force leaves.
+constexpr int kNumLists = kMaxDepth + 4;
+
 // One node of a bucket's split tree, as laid out in HBM (16 B).
 struct NodeDesc {
     uint64_t key_off;  // first key of the node in the bucket-sorted key array

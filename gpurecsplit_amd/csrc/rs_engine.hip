@@ -119,7 +119,6 @@ void device_copy_free(DeviceCopy *d) {
     delete d;
 }
 
-constexpr int kNumLists = kMaxDepth + 4;  // fanout-2 depths, then upper, lower, rotation leaves, brute leaves
 constexpr int kScanItems = 2048;          // items per workgroup of the scan
 
 // ------------------------------------------------------------------ scans
@@ -524,12 +523,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     HIP_CHECK(hipMemsetAsync(counts, 0, NB * 4, s));
     HIP_CHECK(hipMemsetAsync(size_hist, 0, (kMaxBucket + 1) * 4, s));
     HIP_CHECK(hipMemsetAsync(err, 0, 16, s));
-    if (max_nb <= 8192)
-        k_bucket_hist<true><<<tiles.size(), 256, max_nb * 4, s>>>(d_tiles, counts);
-    else
-        k_bucket_hist<false><<<tiles.size(), 256, 0, s>>>(d_tiles, counts);
+    const uint32_t priv_max = (uint32_t)std::min<uint64_t>(max_nb, 8192);
+    k_bucket_hist<<<tiles.size(), 256, (size_t)priv_max * 4, s>>>(d_tiles, counts, priv_max);
     HIP_CHECK(hipGetLastError());
-    k_bucket_meta<<<(NB + 255) / 256, 256, 0, s>>>(counts, NB, memo + kMaxBucket, counts64, node_cnt, size_hist);
+    k_bucket_meta<<<(uint32_t)std::min<uint64_t>((NB + 255) / 256, (uint64_t)c->num_cus * 4), 256, 0, s>>>(
+        counts, NB, memo + kMaxBucket, counts64, node_cnt, size_hist);
     HIP_CHECK(hipGetLastError());
     scan_u64(c, counts, true, NB, acc);
     scan_u64(c, node_cnt, false, NB, node_base);
@@ -563,7 +561,10 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     HIP_CHECK(hipMemsetAsync(cursor, 0, NB * 4, s));
     k_scatter<<<tiles.size(), 256, 0, s>>>(d_tiles, acc, cursor, sorted);
     HIP_CHECK(hipGetLastError());
-    k_sort_buckets<<<NB, st.max_bucket_size > 512 ? 256 : 64, 0, s>>>(sorted, acc, err, rotate ? leaf : 0);
+    uint32_t sort_p = 1;
+    while (sort_p < st.max_bucket_size) sort_p <<= 1;
+    k_sort_buckets<<<NB, st.max_bucket_size > 512 ? 256 : 64, (size_t)sort_p * 8, s>>>(sorted, acc, err,
+                                                                                       rotate ? leaf : 0);
     HIP_CHECK(hipGetLastError());
 
     // ---- 3. node lists
@@ -666,7 +667,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     unsigned long long *evals = c->evals.get<unsigned long long>(kNumKinds);
     HIP_CHECK(hipMemsetAsync(evals, 0, kNumKinds * 8, s));
     EncodeArgs ca{acc, node_base, seeds, node_m, memo, memo + kMaxBucket, NB, P, rotate ? 1 : 0};
-    k_bucket_bits<<<(NB + 3) / 4, 256, 0, s>>>(ca, bits, evals);
+    k_bucket_bits<<<(uint32_t)std::min<uint64_t>((NB + 3) / 4, (uint64_t)c->num_cus * 8), 256, 0, s>>>(ca, bits, evals);
     HIP_CHECK(hipGetLastError());
     scan_u64(c, bits, false, NB, pos);
     uint64_t *d_bidx = c->set_idx.get<uint64_t>(S + 1);

@@ -120,45 +120,49 @@ struct KeyTile {
     uint64_t bbase;        // first global bucket of the set
 };
 
-// Bucket histogram, one workgroup per key tile.  Sets with few buckets count in
-// LDS first (PRIV: every tile's set has at most 8192 buckets).
-template <bool PRIV>
-__global__ __launch_bounds__(256) void k_bucket_hist(const KeyTile *__restrict__ tiles, uint32_t *__restrict__ counts) {
+// Bucket histogram, one workgroup per key tile.  Tiles of sets with at most
+// `priv_max` buckets count in LDS first; larger sets count with global atomics.
+__global__ __launch_bounds__(256) void k_bucket_hist(const KeyTile *__restrict__ tiles, uint32_t *__restrict__ counts,
+                                                     uint32_t priv_max) {
     extern __shared__ uint32_t s_cnt[];
     const KeyTile t = tiles[blockIdx.x];
     const uint64_t *__restrict__ keys = t.keys;
-    if (PRIV) {
+    if (t.nb <= priv_max) {
         for (uint32_t i = threadIdx.x; i < t.nb; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
-    }
-    for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) {
-        const uint64_t b = __umul64hi(keys[2 * i], t.nb);
-        if (PRIV) atomicAdd(&s_cnt[b], 1u);
-        else atomicAdd(&counts[t.bbase + b], 1u);
-    }
-    if (PRIV) {
+        for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) atomicAdd(&s_cnt[__umul64hi(keys[2 * i], t.nb)], 1u);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < t.nb; i += blockDim.x)
             if (s_cnt[i]) atomicAdd(&counts[t.bbase + i], s_cnt[i]);
+    } else {
+        for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x)
+            atomicAdd(&counts[t.bbase + __umul64hi(keys[2 * i], t.nb)], 1u);
     }
 }
 
-// Per bucket: node count (tree_nodes table by bucket size), size histogram,
-// oversize flag.
+// Per bucket: node count (tree_nodes table by bucket size), size histogram
+// (counted in LDS per workgroup, so that the few common sizes do not contend on
+// global atomics), oversize flag.
 __global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict__ counts, uint64_t nb,
                                                      const uint32_t *__restrict__ tree_nodes, uint64_t *__restrict__ counts64,
                                                      uint64_t *__restrict__ nodes, uint32_t *__restrict__ size_hist) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint32_t m = counts[b];
-    counts64[b] = m;
-    if (m >= kMaxBucket) {
-        atomicAdd(&size_hist[kMaxBucket], 1u);
-        nodes[b] = 0;
-        return;
+    __shared__ uint32_t s_hist[kMaxBucket + 1];
+    for (uint32_t i = threadIdx.x; i <= kMaxBucket; i += blockDim.x) s_hist[i] = 0;
+    __syncthreads();
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t m = counts[b];
+        counts64[b] = m;
+        if (m >= kMaxBucket) {
+            atomicAdd(&s_hist[kMaxBucket], 1u);
+            nodes[b] = 0;
+        } else {
+            atomicAdd(&s_hist[m], 1u);
+            nodes[b] = tree_nodes[m];
+        }
     }
-    atomicAdd(&size_hist[m], 1u);
-    nodes[b] = tree_nodes[m];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i <= kMaxBucket; i += blockDim.x)
+        if (s_hist[i]) atomicAdd(&size_hist[i], s_hist[i]);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(const KeyTile *__restrict__ tiles, const uint64_t *__restrict__ bucket_acc,
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void k_scatter(const KeyTile *__restrict__ til
 // values, which no seed can ever separate.
 __global__ __launch_bounds__(256) void k_sort_buckets(uint64_t *__restrict__ sorted, const uint64_t *__restrict__ acc,
                                                       uint32_t *__restrict__ err, uint32_t rot_leaf) {
-    __shared__ uint64_t s[4096];
+    extern __shared__ uint64_t s[];  // next power of two of the largest bucket
     const uint64_t b = blockIdx.x;
     const uint64_t off = acc[b];
     const uint32_t m = (uint32_t)(acc[b + 1] - off);
@@ -234,12 +238,10 @@ __device__ __forceinline__ uint32_t list_of(uint32_t kind, uint32_t depth) {
     return kind == kHigher ? depth : kMaxDepth + kind - 1;
 }
 
-__global__ __launch_bounds__(64) void k_expand(const uint64_t *__restrict__ acc, const uint64_t *__restrict__ node_base,
-                                               uint64_t nb, ExpandArgs a) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint32_t m = (uint32_t)(acc[b + 1] - acc[b]);
-    if (m < 2 || m >= kMaxBucket) return;
+// DFS preorder walk of a bucket's split tree: calls f(list, offset, size, depth,
+// kind) for every node, in slot order.
+template <typename F>
+__device__ __forceinline__ void walk_tree(const SplitParams &P, bool rotate, uint32_t m, F &&f) {
     struct Item {
         uint32_t off;
         uint16_t m;
@@ -248,22 +250,10 @@ __global__ __launch_bounds__(64) void k_expand(const uint64_t *__restrict__ acc,
     Item stack[48];
     int sp = 0;
     stack[sp++] = Item{0, (uint16_t)m, 0};
-    uint32_t slot = (uint32_t)node_base[b];
-    const SplitParams P = a.P;
     while (sp > 0) {
         const Item it = stack[--sp];
-        const uint32_t kind = classify(P, it.m, a.rotate != 0);
-        const uint32_t list = list_of(kind, it.depth);
-        const uint32_t pos = atomicAdd(&a.list_fill[list], 1u);
-        NodeDesc d;
-        d.key_off = acc[b] + it.off;
-        d.slot = slot;
-        d.m = it.m;
-        d.depth = it.depth;
-        d.kind = (uint8_t)kind;
-        a.pool[a.list_base[list] + pos] = d;
-        a.node_m[slot] = it.m;
-        slot++;
+        const uint32_t kind = classify(P, it.m, rotate);
+        f(list_of(kind, it.depth), it.off, (uint32_t)it.m, (uint32_t)it.depth, kind);
         if (it.m <= P.leaf) continue;
         uint32_t unit, fan;
         if (kind == kHigher) {
@@ -282,6 +272,54 @@ __global__ __launch_bounds__(64) void k_expand(const uint64_t *__restrict__ acc,
             if (cs > 1) stack[sp++] = Item{it.off + (uint32_t)c * unit, (uint16_t)cs, (uint8_t)(it.depth + 1)};
         }
     }
+}
+
+// One wavefront per 64 buckets: every bucket's nodes go to their phase lists.
+// A first walk counts each bucket's nodes per list; a wave-wide exclusive scan per
+// list gives each bucket its place, so a wave reserves a list's space with one
+// atomic instead of one per node (millions of nodes contend on a few lists).
+__global__ __launch_bounds__(64) void k_expand(const uint64_t *__restrict__ acc, const uint64_t *__restrict__ node_base,
+                                               uint64_t nb, ExpandArgs a) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t m = b < nb ? (uint32_t)(acc[b + 1] - acc[b]) : 0;
+    const bool active = m >= 2 && m < kMaxBucket;
+    const SplitParams P = a.P;
+    const bool rotate = a.rotate != 0;
+    uint32_t cnt[kNumLists];
+#pragma unroll
+    for (int l = 0; l < kNumLists; l++) cnt[l] = 0;
+    if (active) walk_tree(P, rotate, m, [&](uint32_t list, uint32_t, uint32_t, uint32_t, uint32_t) { cnt[list]++; });
+    uint32_t pos[kNumLists];
+#pragma unroll
+    for (int l = 0; l < kNumLists; l++) {
+        pos[l] = 0;
+        if (!__ballot(cnt[l] != 0)) continue;
+        uint32_t inc = cnt[l];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o);
+            if (lane >= (uint32_t)o) inc += t;
+        }
+        const uint32_t total = __shfl(inc, 63);
+        uint32_t wbase = 0;
+        if (lane == 0) wbase = atomicAdd(&a.list_fill[l], total);
+        pos[l] = __shfl(wbase, 0) + inc - cnt[l];
+    }
+    if (!active) return;
+    uint32_t slot = (uint32_t)node_base[b];
+    const uint64_t koff = acc[b];
+    walk_tree(P, rotate, m, [&](uint32_t list, uint32_t off, uint32_t sz, uint32_t depth, uint32_t kind) {
+        NodeDesc d;
+        d.key_off = koff + off;
+        d.slot = slot;
+        d.m = (uint16_t)sz;
+        d.depth = (uint8_t)depth;
+        d.kind = (uint8_t)kind;
+        a.pool[a.list_base[list] + pos[list]++] = d;
+        a.node_m[slot] = (uint16_t)sz;
+        slot++;
+    });
 }
 
 // ------------------------------------------------------------------ seed search
@@ -888,33 +926,37 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 // whole subtree (memo) + unary bits of every node; algorithmic evaluations per kind.
 __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__restrict__ bits,
                                                      unsigned long long *__restrict__ evals) {
-    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // waves stride over the buckets and keep their evaluation counts in registers:
+    // one atomic per kind per wave at the end, not per bucket
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t n_waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint32_t lane = lane_id();
-    if (b >= a.nb) return;
-    const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
-    if (m < 2) {
-        if (lane == 0) bits[b] = 0;
-        return;
-    }
-    const uint64_t base = a.node_base[b];
-    const uint32_t nn = a.tree_nodes[m];
-    uint64_t un = 0;
     uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0};
-    for (uint32_t j = lane; j < nn; j += 64) {
-        const uint32_t s = a.node_m[base + j];
-        const uint32_t g = a.memo[s] >> 27;
-        const uint64_t x = a.seeds[base + j];
-        un += (x >> g) + 1;
-        const uint32_t kind = classify(a.P, s, a.rotate != 0);
-        const uint64_t e = kind == kLeafRot ? (x / s + 1) * s : (x + 1) * s;
+    for (uint64_t b = wave; b < a.nb; b += n_waves) {
+        const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
+        if (m < 2) {
+            if (lane == 0) bits[b] = 0;
+            continue;
+        }
+        const uint64_t base = a.node_base[b];
+        const uint32_t nn = a.tree_nodes[m];
+        uint64_t un = 0;
+        for (uint32_t j = lane; j < nn; j += 64) {
+            const uint32_t s = a.node_m[base + j];
+            const uint32_t g = a.memo[s] >> 27;
+            const uint64_t x = a.seeds[base + j];
+            un += (x >> g) + 1;
+            const uint32_t kind = classify(a.P, s, a.rotate != 0);
+            const uint64_t e = kind == kLeafRot ? (x / s + 1) * s : (x + 1) * s;
 #pragma unroll
-        for (int k = 0; k < kNumKinds; k++) ev[k] += (uint32_t)k == kind ? e : 0;
+            for (int k = 0; k < kNumKinds; k++) ev[k] += (uint32_t)k == kind ? e : 0;
+        }
+        un = wave_sum64(un);
+        if (lane == 0) bits[b] = (a.memo[m] & 0xFFFF) + un;
     }
-    un = wave_sum64(un);
 #pragma unroll
     for (int k = 0; k < kNumKinds; k++) ev[k] = wave_sum64(ev[k]);
     if (lane == 0) {
-        bits[b] = (a.memo[m] & 0xFFFF) + un;
 #pragma unroll
         for (int k = 0; k < kNumKinds; k++)
             if (ev[k]) atomicAdd(&evals[k], (unsigned long long)ev[k]);
