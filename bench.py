@@ -204,6 +204,7 @@ def batched_4096(gra, ctx, quick: bool, sequential: bool = True) -> dict:
         log(f"4096 sets one by one: {seq_ms:.1f} ms")
     return {"sets": 4096, "keys": n, "leaf": 12, "bucket": 100, "sizes": f"log-uniform [1e3, {hi:.0e}]",
             "ms": ms, "value": ms * 1e6 / n, "unit": "ns/object", "search_ms": st["search_ms"],
+            "kernels": kernel_rooflines([st]),
             "partition_ms": st["partition_ms"], "encode_ms": st["encode_ms"],
             "sequential_plain_ms": seq_ms, "objects_per_s_vs_sequential_plain": seq_ms / ms,
             "bijection_all_sets": bij, "oracle_bytes_sets": sample, "oracle_bytes_equal": oracle_ok}
