@@ -20,9 +20,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rs_internal.h"
@@ -630,6 +632,16 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.err = err;
         a.groups = (uint32_t)tn.groups[kind];
         a.help_cap = (uint32_t)tn.help_cap[kind];
+        // Levels of many small nodes (millions of leaves of many small sets) would
+        // otherwise be bound by the one fresh-node counter (one atomic per node):
+        // tickets take up to 16 nodes, except for the last 8 per wave.
+        a.claim_batch = 1;
+        a.batch_tickets = 0;
+        const uint64_t tail_nodes = (uint64_t)n_waves * 8;
+        if (cnt > tail_nodes) {
+            a.claim_batch = (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, cnt / ((uint64_t)n_waves * 64)));
+            if (a.claim_batch > 1) a.batch_tickets = (uint32_t)((cnt - tail_nodes) / a.claim_batch);
+        }
         bool wide = false, lut = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
@@ -638,11 +650,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             a.field_bits = L.fb;
             a.magic = L.magic;
             wide = L.wide;
-            // one increment table per wave: 4-byte entries, 8-byte ones for 64-bit
-            // counters; tables above 32 KiB per workgroup would cut occupancy
-            const uint32_t entries = ((kind == kUpper ? P.upper : P.lower) + 3) & ~3u;
-            a.lut_stride = entries * (wide ? 2 : 1);
-            lds = (size_t)4 * a.lut_stride * 4;
+            // one increment table per workgroup, indexed by the remapped value
+            // (below the largest node size of the phase): 4-byte entries, 8-byte
+            // ones for 64-bit counters; tables above 32 KiB would cut occupancy
+            a.lut_entries = kind == kUpper ? P.upper : P.lower;
+            lds = (size_t)a.lut_entries * (wide ? 8 : 4);
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
@@ -741,12 +753,6 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         ef_words[i][1] = ef_upper_words(nb, ef.u_cum_keys, ef.l_cum_keys);
         ef_words[i][2] = ef_upper_words(nb, ef.u_position, ef.l_position);
         ef_words[i][3] = enc[i].jump_words;
-        if (!so) {
-            ef.lower_bits.assign(ef_words[i][0], 0);
-            ef.upper_cum.assign(ef_words[i][1], 0);
-            ef.upper_pos.assign(ef_words[i][2], 0);
-            ef.jump.assign(ef_words[i][3], 0);
-        }
         tot_lower += ef_words[i][0];
         tot_uc += ef_words[i][1];
         tot_up += ef_words[i][2];
@@ -805,35 +811,37 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         if (h[16] & 4) throw BuildError(RS_ERR_EF_OVERFLOW, "Elias-Fano jump offset does not fit 16 bits");
         return {};
     }
-    std::vector<uint64_t> h_rice(rice_total), h_lower(tot_lower), h_uc(tot_uc), h_up(tot_up), h_jump(tot_jump);
-    HIP_CHECK(hipMemcpyAsync(h_rice.data(), rice, rice_total * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(h_lower.data(), d_lower, tot_lower * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(h_uc.data(), d_uc, tot_uc * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(h_up.data(), d_up, tot_up * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(h_jump.data(), d_jump, tot_jump * 8, hipMemcpyDeviceToHost, s));
+    // One copy per array, all sets at once, into uninitialised host memory (a
+    // zero-filled staging vector would cost as much as the copy at 600M keys).
+    std::unique_ptr<uint64_t[]> h_rice(new uint64_t[rice_total + 1]), h_lower(new uint64_t[tot_lower + 1]),
+        h_uc(new uint64_t[tot_uc + 1]), h_up(new uint64_t[tot_up + 1]), h_jump(new uint64_t[tot_jump + 1]);
+    HIP_CHECK(hipMemcpyAsync(h_rice.get(), rice, rice_total * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_lower.get(), d_lower, tot_lower * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_uc.get(), d_uc, tot_uc * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_up.get(), d_up, tot_up * 8, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h_jump.get(), d_jump, tot_jump * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
     DeviceCopy *dc = nullptr;
     if (device_copy && S == 1) {
         dc = new DeviceCopy();
         dc->device = c->device;
-        const EliasFano &ef = efs[0];
+        dc->n_rice = rice_total;
+        dc->n_lower = ef_words[0][0];
+        dc->n_uc = ef_words[0][1];
+        dc->n_up = ef_words[0][2];
+        dc->n_jump = ef_words[0][3];
         HIP_CHECK(hipMalloc(&dc->rice, (rice_total + 2) * 8));
-        HIP_CHECK(hipMalloc(&dc->lower, (ef.lower_bits.size() + 1) * 8));
-        HIP_CHECK(hipMalloc(&dc->upper_c, (ef.upper_cum.size() + 1) * 8));
-        HIP_CHECK(hipMalloc(&dc->upper_p, (ef.upper_pos.size() + 1) * 8));
-        HIP_CHECK(hipMalloc(&dc->jump, ef.jump.size() * 8 + 8));
+        HIP_CHECK(hipMalloc(&dc->lower, (dc->n_lower + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->upper_c, (dc->n_uc + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->upper_p, (dc->n_up + 1) * 8));
+        HIP_CHECK(hipMalloc(&dc->jump, dc->n_jump * 8 + 8));
         HIP_CHECK(hipMalloc(&dc->memo, kMaxBucket * 4));
         HIP_CHECK(hipMemcpyAsync(dc->rice, rice, (rice_total + 2) * 8, hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(dc->lower, d_lower, (ef.lower_bits.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(dc->upper_c, d_uc, (ef.upper_cum.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (ef.upper_pos.size() + 1) * 8, hipMemcpyDeviceToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, ef.jump.size() * 8 + 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->lower, d_lower, (dc->n_lower + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->upper_c, d_uc, (dc->n_uc + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->upper_p, d_up, (dc->n_up + 1) * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(dc->jump, d_jump, dc->n_jump * 8 + 8, hipMemcpyDeviceToDevice, s));
         HIP_CHECK(hipMemcpyAsync(dc->memo, memo, kMaxBucket * 4, hipMemcpyDeviceToDevice, s));
-        dc->n_rice = rice_total;
-        dc->n_lower = ef.lower_bits.size();
-        dc->n_uc = ef.upper_cum.size();
-        dc->n_up = ef.upper_pos.size();
-        dc->n_jump = ef.jump.size();
     }
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[16] & 4) {
@@ -850,8 +858,10 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     st.encode_ms = ev_ms(c, e_search_end, e_enc_end);
     st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 
-    std::vector<rs_mphf *> out(S);
-    for (size_t i = 0; i < S; i++) {
+    // One handle per set, each with its own copy of its words; many sets are
+    // assembled on several host threads.
+    std::vector<rs_mphf *> out(S, nullptr);
+    auto make = [&](size_t i) {
         rs_mphf *r = new rs_mphf();
         r->leaf = leaf;
         r->rotate = flags & (kFlagRotate | kFlagRemap64);
@@ -859,21 +869,48 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         r->keys_count = set_off[i + 1] - set_off[i];
         r->nbuckets = nbs[i];
         r->rice_words = rice_words[i];
-        r->rice.assign(h_rice.begin() + enc[i].rice_word, h_rice.begin() + enc[i].rice_word + rice_words[i]);
+        r->rice.reserve(rice_words[i] + 2);
+        r->rice.assign(h_rice.get() + enc[i].rice_word, h_rice.get() + enc[i].rice_word + rice_words[i]);
         r->rice.resize(rice_words[i] + 2, 0);
         EliasFano &ef = efs[i];
-        std::copy(h_lower.begin() + enc[i].lower_word, h_lower.begin() + enc[i].lower_word + ef.lower_bits.size(),
-                  ef.lower_bits.begin());
-        std::copy(h_uc.begin() + enc[i].uc_word, h_uc.begin() + enc[i].uc_word + ef.upper_cum.size(), ef.upper_cum.begin());
-        std::copy(h_up.begin() + enc[i].up_word, h_up.begin() + enc[i].up_word + ef.upper_pos.size(), ef.upper_pos.begin());
-        std::copy(h_jump.begin() + enc[i].jump_word, h_jump.begin() + enc[i].jump_word + ef.jump.size(), ef.jump.begin());
+        ef.lower_bits.assign(h_lower.get() + enc[i].lower_word, h_lower.get() + enc[i].lower_word + ef_words[i][0]);
+        ef.upper_cum.assign(h_uc.get() + enc[i].uc_word, h_uc.get() + enc[i].uc_word + ef_words[i][1]);
+        ef.upper_pos.assign(h_up.get() + enc[i].up_word, h_up.get() + enc[i].up_word + ef_words[i][2]);
+        ef.jump.assign(h_jump.get() + enc[i].jump_word, h_jump.get() + enc[i].jump_word + ef_words[i][3]);
         r->ef = std::move(ef);
         r->memo = &hmemo;
         r->P = P;
         r->stats = st;
-        if (i == 0) r->dev = dc;
         out[i] = r;
+    };
+    const size_t n_threads = S >= 256 ? std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())) : 1;
+    bool failed = false;
+    if (n_threads <= 1) {
+        try {
+            for (size_t i = 0; i < S; i++) make(i);
+        } catch (const std::bad_alloc &) {
+            failed = true;
+        }
+    } else {
+        std::vector<std::thread> pool;
+        std::vector<char> bad(n_threads, 0);
+        for (size_t t = 0; t < n_threads; t++)
+            pool.emplace_back([&, t] {
+                try {
+                    for (size_t i = t; i < S; i += n_threads) make(i);
+                } catch (const std::bad_alloc &) {
+                    bad[t] = 1;
+                }
+            });
+        for (std::thread &th : pool) th.join();
+        for (char b : bad) failed |= b != 0;
     }
+    if (failed) {
+        for (rs_mphf *r : out) delete r;
+        if (dc) device_copy_free(dc);
+        throw std::bad_alloc();
+    }
+    out[0]->dev = dc;
     return out;
 }
 

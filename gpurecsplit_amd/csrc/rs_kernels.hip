@@ -338,8 +338,13 @@ struct SearchArgs {
     uint32_t field_bits;  // packed counter field width
     uint32_t magic;       // ceil(2^20 / split_unit)
     uint32_t upper;       // upper aggregation bound (fanout-2 split size)
-    uint32_t lut_stride;  // LDS words per wave for the aggregation increment table
+    uint32_t lut_entries;  // entries of the aggregation increment table (largest node size of the phase)
     uint32_t help_cap;    // helpers join only nodes with fewer waves than this (0: no cap)
+    // Fresh nodes are claimed by tickets from ctl[0]: tickets below batch_tickets
+    // take claim_batch consecutive nodes each, later tickets one node each (the
+    // last nodes of the level, so no wave holds unstarted nodes at the end).
+    uint32_t claim_batch;
+    uint32_t batch_tickets;
     // Time limit: 0 = none; otherwise waves leave the search once the device wall
     // clock passes *clock0 + limit_ticks, setting bit 3 of *err.
     const unsigned long long *clock0;
@@ -396,12 +401,6 @@ __device__ __forceinline__ void sload_keys8(const uint64_t *p, uint64_t (&k)[8])
 #pragma unroll
     for (int j = 0; j < 8; j++) k[j] = ((uint64_t)r[2 * j + 1] << 32) | r[2 * j];
 }
-__device__ __forceinline__ uint64_t sload_key(const uint64_t *p) {
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-    u32x2 r;
-    __asm__ volatile("s_load_dwordx2 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=&s"(r) : "s"(p) : "memory");
-    return ((uint64_t)r[1] << 32) | r[0];
-}
 
 // Streams keys kp[begin, end) (kp wave-uniform) through scalar registers, eight
 // keys per scalar load.  The wait of each load also waits for the LDS reads of
@@ -421,7 +420,16 @@ __device__ __forceinline__ void stream_keys_scalar(const uint64_t *__restrict__ 
 #pragma unroll
         for (int j = 0; j < 8; j++) f(k[j]);
     }
-    for (; i < end; i++) f(sload_key(kp + i));
+    if (i < end) {
+        // the last 1..7 keys in one load: the key array carries 8 entries of
+        // padding, so kp[end + 6] is inside it; keys from `end` on are not used
+        uint64_t k[8];
+        sload_keys8(kp + i, k);
+        const uint32_t rem = end - i;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((uint32_t)j < rem) f(k[j]);
+    }
 }
 
 // Candidate seeds of one group: lane-u candidate = base + off[u] (mod 2^64), base
@@ -520,7 +528,7 @@ __device__ __forceinline__ uint64_t group_higher(const uint64_t *__restrict__ kp
 // `fb` bits (2^fb >= SPLIT + 2), the last part uncounted; a carry out of any field
 // forces another count negative, so equality of the counted fields is
 // equality of all counts.  With LUT, the increment 1 << (fb * part) of every
-// remapped value r < m is read from a per-wave LDS table instead of computed.
+// remapped value r < m is read from the workgroup's LDS table instead of computed.
 template <int U, bool WIDE, bool LUT, bool SK, bool R64>
 __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                uint32_t fb, uint32_t magic, uint64_t T, uint64_t cmask,
@@ -644,17 +652,26 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
-    // Aggregation increment table of this wave: 4-byte entries, or 8-byte entries
-    // for 64-bit counters (WIDE); lut_stride counts 4-byte words and is even.
+    // Aggregation increment table, one per workgroup: entry r (a remapped value)
+    // holds 1 << (fb * floor(r / split_unit)).  It depends on r alone, not on the
+    // node, so it is written once before the first claim.  4-byte entries, 8-byte
+    // ones for 64-bit counters (WIDE).
     constexpr uint32_t kLutShift = WIDE ? 3 : 2;
-    uint32_t *lut = s_lut + (threadIdx.x >> 6) * a.lut_stride;
+    if constexpr (LUT && (KIND == kUpper || KIND == kLower)) {
+        for (uint32_t r = threadIdx.x; r < a.lut_entries; r += blockDim.x) {
+            if (WIDE) ((uint64_t *)s_lut)[r] = 1ull << (a.field_bits * (r / a.split_unit));
+            else s_lut[r] = 1u << (a.field_bits * (r / a.split_unit));
+        }
+        __syncthreads();
+    }
     // table start in entries, in the upper half (the low 32 bits of a flat LDS
     // address are its LDS offset)
-    const uint32_t lut_hi = uni(
-        (((uint32_t)(uintptr_t)s_lut >> kLutShift) + (threadIdx.x >> 6) * (a.lut_stride >> (kLutShift - 2))) << 16);
+    const uint32_t lut_hi = uni(((uint32_t)(uintptr_t)s_lut >> kLutShift) << 16);
     const uint32_t two = kLutShift;
     int node = -1;
     bool helping = false;
+    bool fresh = false;                     // node came from the fresh-node tickets
+    uint32_t batch_next = 0, batch_end = 0;  // claimed fresh nodes not yet started
     uint32_t scan = wave;
     uint64_t work = 0;
     // cached per-node values (wave-uniform)
@@ -664,12 +681,27 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
 
     for (;;) {
         if (node < 0) {
+            fresh = false;
             if (!helping) {
-                uint32_t c = 0;
-                if (lane == 0) c = atomicAdd(&a.ctl[0], 1u);
-                c = uni(c);
-                if (c < a.n_nodes) node = (int)c;
-                else helping = true;
+                if (batch_next >= batch_end) {
+                    uint32_t t = 0;
+                    if (lane == 0) t = atomicAdd(&a.ctl[0], 1u);
+                    t = uni(t);
+                    if (t < a.batch_tickets) {
+                        batch_next = t * a.claim_batch;
+                        batch_end = batch_next + a.claim_batch;
+                    } else {
+                        const uint32_t c = a.batch_tickets * a.claim_batch + (t - a.batch_tickets);
+                        batch_next = c;
+                        batch_end = c < a.n_nodes ? c + 1 : c;
+                    }
+                }
+                if (batch_next < batch_end) {
+                    node = (int)batch_next++;
+                    fresh = true;
+                } else {
+                    helping = true;
+                }
             }
             if (helping) {
                 int found = -1;
@@ -721,17 +753,10 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 for (uint32_t p = 0; p + 1 < fan; p++) T |= (uint64_t)su << (p * a.field_bits);
                 const uint32_t w = (fan - 1) * a.field_bits;
                 cmask = w >= 64 ? ~0ull : ((1ull << w) - 1);
-                if (LUT) {
-                    if (WIDE)
-                        for (uint32_t r = lane; r < m; r += 64)
-                            ((uint64_t *)lut)[r] = 1ull << (a.field_bits * (r / su));
-                    else
-                        for (uint32_t r = lane; r < m; r += 64) lut[r] = 1u << (a.field_bits * (r / su));
-                    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                }
             } else if (KIND == kLeafRot) {
-                uint32_t odd = 0;
-                for (uint32_t i = 0; i < m; i++) odd += (uint32_t)(keys[key_off + i] & 1);
+                // m <= 24 keys: lane i reads key i, a ballot counts the odd ones
+                const bool odd_key = lane < m && (keys[key_off + lane] & 1) != 0;
+                const uint32_t odd = (uint32_t)__popcll(__ballot(odd_key));
                 aux0 = m - odd;  // nl
                 aux1 = odd;      // nr
             } else {
@@ -759,13 +784,19 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 break;
             }
         }
-        unsigned long long chunk = 0, res = 0;
-        if (lane == 0) {
-            chunk = atomicAdd(&a.state[node].next_chunk, 1ull);
-            res = ld_relaxed(&a.state[node].result);
+        unsigned long long chunk = 0, res = kNone;
+        if (fresh) {
+            // chunk 0 belongs to the wave that starts the node (helpers claim from
+            // 1 on), so it needs no claim
+            fresh = false;
+        } else {
+            if (lane == 0) {
+                chunk = atomicAdd(&a.state[node].next_chunk, 1ull);
+                res = ld_relaxed(&a.state[node].result);
+            }
+            chunk = uni64(chunk);
+            res = uni64(res);
         }
-        chunk = uni64(chunk);
-        res = uni64(res);
         const uint64_t unit0 = chunk * units_per_chunk;
         if (unit0 * scale >= res || unit0 * scale >= kSeedLimit) {
             if (unit0 * scale >= kSeedLimit && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)kSeedLimit);
@@ -795,7 +826,16 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             else hit = group_leaf_brute<U, SK, R64>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
-        if (hit != kNone && lane == 0) atomicMin(&a.state[node].result, (unsigned long long)hit);
+        if (hit != kNone) {
+            // every chunk this wave could still claim lies above this one, so
+            // above the hit: leave the node
+            if (lane == 0) {
+                atomicMin(&a.state[node].result, (unsigned long long)hit);
+                __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicSub(&a.state[node].waves, 1u);
+            }
+            node = -1;
+        }
     }
     if (lane == 0) {
         __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -807,7 +847,7 @@ __global__ __launch_bounds__(256) void k_state_init(NodeState *st, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         st[i].result = kNone;
-        st[i].next_chunk = 0;
+        st[i].next_chunk = 1;  // chunk 0 is taken by the wave that starts the node
         st[i].waves = 0;
     }
 }
