@@ -684,6 +684,8 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             fresh = false;
             if (!helping) {
                 if (batch_next >= batch_end) {
+                    // tickets come from one counter, so only resident waves hold
+                    // nodes (workgroups beyond what fits start as others leave)
                     uint32_t t = 0;
                     if (lane == 0) t = atomicAdd(&a.ctl[0], 1u);
                     t = uni(t);
@@ -705,29 +707,45 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             }
             if (helping) {
                 int found = -1;
-                // Among the first 64 running entries with work left, join the node
-                // with the fewest waves (spreads helpers evenly over the nodes
-                // still open; a node's chunk overshoot grows with its waves).
-                for (uint32_t k = 0; k < a.n_waves && found < 0; k += 64) {
-                    uint32_t idx = scan + k + lane;
-                    while (idx >= a.n_waves) idx -= a.n_waves;
-                    int cand = -1;
-                    if (k + lane < a.n_waves) cand = __hip_atomic_load(&a.running[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    bool ok = cand >= 0 && has_work(&a.state[cand], units_per_chunk, scale);
-                    uint32_t w = 0;
-                    if (ok) {
-                        w = __hip_atomic_load(&a.state[cand].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = a.help_cap == 0 || w < a.help_cap;
+                // Among the running entries of a window of 256 (four per lane, the
+                // loads of a window issued together) with work left, join the node
+                // with the fewest waves (spreads helpers evenly over the nodes still
+                // open; a node's chunk overshoot grows with its waves).
+                for (uint32_t k = 0; k < a.n_waves && found < 0; k += 256) {
+                    int cand[4];
+                    uint32_t idx[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        idx[j] = scan + k + 4 * lane + j;
+                        while (idx[j] >= a.n_waves) idx[j] -= a.n_waves;
+                        cand[j] = k + 4 * lane + j < a.n_waves
+                                      ? __hip_atomic_load(&a.running[idx[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : -1;
                     }
-                    const unsigned long long msk = __ballot(ok);
-                    if (msk) {
-                        uint32_t key = 0xFFFFFFFFu;
-                        if (ok) key = (min(w, 0xFFFFu) << 16) | lane;
+                    uint32_t best = 0xFFFFFFFFu;
+                    int best_cand = -1;
+                    uint32_t best_idx = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        bool ok = cand[j] >= 0 && has_work(&a.state[cand[j]], units_per_chunk, scale);
+                        uint32_t w = 0;
+                        if (ok) {
+                            w = __hip_atomic_load(&a.state[cand[j]].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            ok = a.help_cap == 0 || w < a.help_cap;
+                        }
+                        if (ok && min(w, 0xFFFFu) < best) {
+                            best = min(w, 0xFFFFu);
+                            best_cand = cand[j];
+                            best_idx = idx[j];
+                        }
+                    }
+                    if (__ballot(best_cand >= 0)) {
+                        uint32_t key = best_cand >= 0 ? (best << 16) | lane : 0xFFFFFFFFu;
 #pragma unroll
                         for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o));
                         const int l = (int)(uni(key) & 0xFFFF);
-                        found = (int)uni((uint32_t)__shfl(cand, l));
-                        scan = uni(__shfl(idx, l));
+                        found = (int)uni((uint32_t)__shfl(best_cand, l));
+                        scan = uni(__shfl(best_idx, l));
                     }
                 }
                 if (found < 0) break;
