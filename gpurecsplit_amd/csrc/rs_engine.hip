@@ -301,8 +301,9 @@ struct rs_ctx {
     std::mutex mu;
     DevBuf keys_in, counts, counts64, cursor, acc, node_cnt, node_base, size_hist, scan_tmp, sorted, pool, list_base,
         list_fill, node_m, seeds, state, running, ctl, err, bits, pos, evals, work, rice, ef_scalars, ef_lower, ef_uc,
-        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock, str_bytes, str_off, tune_keys;
+        ef_up, ef_jump, tiles, set_idx, set_gather, set_enc, bit_off, clock, str_bytes, str_off, tune_keys, wave_clock;
     double time_limit_s = 0;  // 0: none
+    bool wave_times = false;  // RS_WAVE_TIMES: record search waves' start and exit times
     int wall_khz = 0;         // device wall-clock rate
     uint32_t *memo[kMaxLeaf + 1] = {};
     uint32_t *h_pinned = nullptr;  // small pinned staging area
@@ -407,6 +408,8 @@ static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bo
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
+    } else if (KIND == kLeafRot && a.leaf <= 14) {
+        k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
     } else {
         k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     }
@@ -461,6 +464,48 @@ struct SerialOut {
     size_t cap;
     size_t *len;
 };
+
+// Idle share of every search launch's tail from the waves' start and exit times
+// (RS_WAVE_TIMES): per level, busy = sum of the waves' times in the search loop /
+// (most waves resident at once * (last exit - first start)); tail = last exit -
+// median exit.  Kinds with several levels (fanout-2 depths) report the
+// duration-weighted busy share and the longest tail.
+static void wave_time_stats(rs_ctx *c, const unsigned long long *d_clock, uint32_t n_waves, const uint64_t *count,
+                            rs_build_stats &st) {
+    std::vector<unsigned long long> t((size_t)kNumLists * 2 * n_waves);
+    HIP_CHECK(hipMemcpyAsync(t.data(), d_clock, t.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double busy_w[kNumKinds] = {}, dur_w[kNumKinds] = {};
+    const double ms_per_tick = 1.0 / c->wall_khz;
+    for (int l = 0; l < kNumLists; l++) {
+        if (!count[l]) continue;
+        const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
+        std::vector<std::pair<unsigned long long, int>> ev;
+        std::vector<unsigned long long> exits;
+        unsigned long long t0 = ~0ull, t1 = 0;
+        double inside = 0;
+        for (uint32_t w = 0; w < n_waves; w++) {
+            const unsigned long long a = t[((size_t)l * n_waves + w) * 2], b = t[((size_t)l * n_waves + w) * 2 + 1];
+            if (!a || b < a) continue;
+            t0 = std::min(t0, a);
+            t1 = std::max(t1, b);
+            inside += (double)(b - a);
+            exits.push_back(b);
+            ev.push_back({a, 1});
+            ev.push_back({b, -1});
+        }
+        if (exits.empty() || t1 <= t0) continue;
+        std::sort(ev.begin(), ev.end());
+        int cur = 0, resident = 0;
+        for (auto &e : ev) resident = std::max(resident, cur += e.second);
+        std::nth_element(exits.begin(), exits.begin() + exits.size() / 2, exits.end());
+        const double dur = (double)(t1 - t0);
+        busy_w[kind] += inside / resident;
+        dur_w[kind] += dur;
+        st.search_tail_ms[kind] = std::max(st.search_tail_ms[kind], (t1 - exits[exits.size() / 2]) * ms_per_tick);
+    }
+    for (int k = 0; k < kNumKinds; k++) st.search_busy[k] = dur_w[k] > 0 ? busy_w[k] / dur_w[k] : 0;
+}
 
 // Builds one MPHF per key set; set s has set_n[s] keys at set_keys[s] (device).
 // All sets share one leaf size and bucket size; their buckets, tree nodes and
@@ -602,6 +647,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     uint32_t *ctl = c->ctl.get<uint32_t>(16);
     unsigned long long *work = c->work.get<unsigned long long>(kNumLists);
     HIP_CHECK(hipMemsetAsync(work, 0, kNumLists * 8, s));
+    unsigned long long *wave_clock = nullptr;
+    if (c->wave_times && !t_arena) {
+        wave_clock = c->wave_clock.get<unsigned long long>((size_t)kNumLists * 2 * n_waves);
+        HIP_CHECK(hipMemsetAsync(wave_clock, 0, (size_t)kNumLists * 2 * n_waves * 8, s));
+    }
     const AggrLayout up_l = aggr_layout(P.lower, P.upper / P.lower);
     const AggrLayout lo_l = aggr_layout(P.leaf, P.lower / P.leaf);
     struct PhaseEvents {
@@ -630,6 +680,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.clock0 = clock0;
         a.limit_ticks = limit_ticks;
         a.err = err;
+        a.wave_clock = wave_clock ? wave_clock + (size_t)l * 2 * n_waves : nullptr;
         a.groups = (uint32_t)tn.groups[kind];
         a.help_cap = (uint32_t)tn.help_cap[kind];
         // Levels of many small nodes (millions of leaves of many small sets) would
@@ -697,6 +748,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
     for (int l = 0; l < kNumLists; l++)
         st.executed_evals[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += h64[16 + l];
+    if (wave_clock) wave_time_stats(c, wave_clock, n_waves, cen.count, st);
     if (h[16] & 8) throw BuildError(RS_ERR_TIMEOUT, "the build passed the context's time limit during the seed search");
     if (h[16] & 2) throw BuildError(RS_ERR_SEED_OVERFLOW, "a node search passed the 2^44 seed limit");
 
@@ -1136,6 +1188,7 @@ int rs_ctx_create(int device, rs_ctx **out) {
         HIP_CHECK(hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device));
         if (c->wall_khz <= 0) c->wall_khz = 100000;  // s_memrealtime: 100 MHz on gfx9 parts
         if (const char *tl = getenv("RS_TIME_LIMIT")) c->time_limit_s = std::max(0.0, atof(tl));
+        if (const char *wt = getenv("RS_WAVE_TIMES")) c->wave_times = atoi(wt) != 0;
         {
             // the allocation-free entry point's scratch: this device's address, once
             std::lock_guard<std::mutex> g(g_scratch_mu);

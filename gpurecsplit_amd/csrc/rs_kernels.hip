@@ -350,6 +350,9 @@ struct SearchArgs {
     const unsigned long long *clock0;
     unsigned long long limit_ticks;
     uint32_t *err;
+    // Optional (nullptr: off): [2 * wave] start and [2 * wave + 1] exit of every
+    // wave on the device wall clock, for the idle share of the launch's tail.
+    unsigned long long *wave_clock;
 };
 
 // Build start on the device wall clock (100 MHz constant clock), the reference
@@ -571,7 +574,7 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
 // The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
-template <int U, bool SK, bool R64>
+template <int U, bool SK, bool R64, bool RB>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
                                                    uint32_t nr, uint64_t u0) {
     const uint64_t S = c_start_seed[kSeedLeaf];
@@ -587,24 +590,70 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
         each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
         each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
     }
-    // Candidates whose two masks are collision-free are tested in index order, one
-    // at a time across the wave: lane r tests rotation r, so one step covers all L
-    // rotations and the lowest fitting rotation is the lowest set bit of a ballot.
-    const uint32_t r = lane_id() < L ? lane_id() : 0;
+    const uint32_t lane = lane_id();
+    if constexpr (!RB) {
+        // Large leaves: few candidates per group (under 4 at L = 16), tested in
+        // index order one at a time across the wave: lane r tests rotation r.
+        const uint32_t r = lane < L ? lane : 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            unsigned long long cand = __ballot((uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr);
+            while (cand) {
+                const int l = __ffsll((long long)cand) - 1;
+                cand &= cand - 1;
+                const uint32_t bl = (uint32_t)__builtin_amdgcn_readlane((int)ml[u], l);
+                const uint32_t br = (uint32_t)__builtin_amdgcn_readlane((int)mr[u], l);
+                const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
+                const unsigned long long fit = __ballot(lane < L && (bl | rot) == full);
+                if (fit) return (u0 + (uint64_t)(u * 64 + l)) * L + (uint64_t)(__ffsll((long long)fit) - 1);
+            }
+        }
+        return kNone;
+    } else {
+    // Small leaves (RB, L <= 14; 13 candidates per group of 256 at L = 12): candidates whose two
+    // masks are collision-free go to a per-wave LDS list in unit order (u, then
+    // lane): left mask | unit << 24 (L <= 24, unit < 256) and the right mask.
+    // Each step then tests floor(64 / L) consecutive candidates: lane i tests
+    // rotation i mod L of candidate i / L, so the lowest set bit of the ballot is
+    // the lowest fitting rotation of the first candidate that has one.
+    __shared__ uint32_t s_cand[4][2][U * 64];  // one list per wave of the workgroup
+    uint32_t *cl = s_cand[threadIdx.x >> 6][0], *cr = s_cand[threadIdx.x >> 6][1];
+    uint32_t count = 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        unsigned long long cand = __ballot((uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr);
-        while (cand) {
-            const int l = __ffsll((long long)cand) - 1;
-            cand &= cand - 1;
-            const uint32_t bl = (uint32_t)__builtin_amdgcn_readlane((int)ml[u], l);
-            const uint32_t br = (uint32_t)__builtin_amdgcn_readlane((int)mr[u], l);
-            const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
-            const unsigned long long fit = __ballot(lane_id() < L && (bl | rot) == full);
-            if (fit) return (u0 + (uint64_t)(u * 64 + l)) * L + (uint64_t)(__ffsll((long long)fit) - 1);
+        const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
+        const unsigned long long bal = __ballot(ok);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        if (ok) {
+            cl[count + below] = ml[u] | ((uint32_t)(u * 64) + lane) << 24;
+            cr[count + below] = mr[u];
+        }
+        count += (uint32_t)__popcll(bal);
+    }
+    if (count == 0) return kNone;
+    // the list is read back by other lanes of this wave: wait for the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t per_step = 64 / L;
+    const uint32_t g = (lane * ((65536 + L - 1) / L)) >> 16;  // lane / L (lane < 64, L <= 24)
+    const uint32_t r = lane - g * L;
+    for (uint32_t base = 0; base < count; base += per_step) {
+        const uint32_t idx = base + g;
+        const bool valid = g < per_step && idx < count;
+        const uint32_t el = valid ? cl[idx] : 0;
+        const uint32_t bl = el & 0xFFFFFFu, br = valid ? cr[idx] : 0;
+        const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
+        const unsigned long long fit = __ballot(valid && (bl | rot) == full);
+        if (fit) {
+            const uint32_t first = (uint32_t)__ffsll((long long)fit) - 1;
+            const uint32_t unit = (uint32_t)__builtin_amdgcn_readlane((int)(el >> 24), (int)first);
+            const uint32_t rr = first - (first / L) * L;
+            return (u0 + (uint64_t)unit) * L + (uint64_t)rr;
         }
     }
     return kNone;
+    }
 }
 
 // Leaf without rotation (SIMDRecSplit.hpp:510-599): lowest x whose m keys land on
@@ -645,11 +694,14 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64>
-__global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
+// RB (rotation leaves of at most 14 keys): the candidate test batched through LDS,
+// whose extra registers are held to five waves per SIMD.
+template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false>
+__global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (a.wave_clock && lane == 0) a.wave_clock[2 * wave] = (unsigned long long)wall_clock64();
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
     // Aggregation increment table, one per workgroup: entry r (a remapped value)
@@ -671,6 +723,8 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
     int node = -1;
     bool helping = false;
     bool fresh = false;                     // node came from the fresh-node tickets
+    uint32_t span_next = 1;                 // chunks to claim at once on this node
+    uint32_t span_cap = 1;                  // at most 2^19 key evaluations per claim, at most 128 chunks
     uint32_t batch_next = 0, batch_end = 0;  // claimed fresh nodes not yet started
     uint32_t scan = wave;
     uint64_t work = 0;
@@ -752,6 +806,8 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
                 node = found;
             }
             node = (int)uni((uint32_t)node);
+            span_next = 1;
+            span_cap = 0;  // set below from the node's size
             if (lane == 0) {
                 __hip_atomic_store(&a.running[wave], node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 atomicAdd(&a.state[node].waves, 1u);
@@ -759,6 +815,13 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             const NodeDesc d = a.nodes[node];
             key_off = uni64(d.key_off);
             m = uni(d.m);
+            {
+                // the power of two at most 2^19 / (key evaluations per chunk), 1..128
+                const uint32_t e = uni((uint32_t)units_per_chunk * m);  // < 2^22
+                const int lg = 32 - __clz(e);                           // e < 2^lg
+                span_cap = lg >= 19 ? 1u : min(128u, 1u << (19 - lg));
+                span_cap = uni(span_cap);
+            }
             if (KIND == kHigher) {
                 sconst = c_start_seed[uni(d.depth)];
                 aux0 = higher_split(SplitParams{a.leaf, 0, a.upper}, m);           // split
@@ -803,17 +866,25 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             }
         }
         unsigned long long chunk = 0, res = kNone;
+        uint32_t span = 1;  // consecutive chunks taken by this claim
         if (fresh) {
             // chunk 0 belongs to the wave that starts the node (helpers claim from
             // 1 on), so it needs no claim
             fresh = false;
         } else {
+            // A node shared by many waves would be bound by its one chunk counter
+            // (thousands of waves on the last heavy nodes of a level): the span
+            // grows with the node's waves seen at the previous claim, 1 below 32.
+            span = span_next;
+            unsigned long long w = 0;
             if (lane == 0) {
-                chunk = atomicAdd(&a.state[node].next_chunk, 1ull);
+                chunk = atomicAdd(&a.state[node].next_chunk, (unsigned long long)span);
                 res = ld_relaxed(&a.state[node].result);
+                w = __hip_atomic_load(&a.state[node].waves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             chunk = uni64(chunk);
             res = uni64(res);
+            span_next = uni(min(max(uni((uint32_t)w) >> 4, 1u), span_cap));
         }
         const uint64_t unit0 = chunk * units_per_chunk;
         if (unit0 * scale >= res || unit0 * scale >= kSeedLimit) {
@@ -827,7 +898,7 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
         }
         const uint64_t *kp = keys + key_off;
         uint64_t hit = kNone;
-        for (uint32_t g = 0; g < a.groups; g++) {
+        for (uint32_t g = 0; g < a.groups * span; g++) {
             const uint64_t u0 = unit0 + (uint64_t)g * U * 64;
             if (g > 0) {
                 unsigned long long r = 0;
@@ -840,7 +911,7 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
             else if (KIND == kUpper || KIND == kLower)
                 hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
                                                         u0);
-            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64>(kp, m, aux0, aux1, u0);
+            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64, RB>(kp, m, aux0, aux1, u0);
             else hit = group_leaf_brute<U, SK, R64>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
@@ -858,6 +929,7 @@ __global__ __launch_bounds__(256) void k_search(const uint64_t *__restrict__ key
     if (lane == 0) {
         __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(a.work, (unsigned long long)work);
+        if (a.wave_clock) a.wave_clock[2 * wave + 1] = (unsigned long long)wall_clock64();
     }
 }
 

@@ -86,6 +86,14 @@ typedef struct rs_build_stats {
     uint64_t rice_bits;
     uint32_t max_bucket_size;
     uint32_t search_waves;    /* persistent waves per search launch */
+    /* Per kind (same index), filled when the context was created with the
+     * environment variable RS_WAVE_TIMES=1 (else 0): every search wave records its
+     * start and exit on the device clock.  search_busy = share of the resident
+     * waves' time, from the first start to the last exit of the launch, spent
+     * inside the search loop (1 - idle share of the launch's tail);
+     * search_tail_ms = time from the median wave's exit to the last exit. */
+    double search_busy[8];
+    double search_tail_ms[8];
 } rs_build_stats;
 
 /* Context bound to one GPU (device ordinal as seen by HIP). */
