@@ -16,6 +16,10 @@ and, as extra keys, the other workloads measured on the same GPU:
   batched_4096        4096 sets, log-uniform sizes in [1e3, 1e6] (~600M keys), l = 12,
                       b = 100, one batch, vs the same 4096 sets built one by one
   served              a scaled served stream through the construction server
+  served_half_rate    the same stream at half the arrival rate (latency without
+                      queueing at the server's capacity)
+  level_tails         busy share and exit tail of every search launch (one more,
+                      untimed build with RS_WAVE_TIMES=1)
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--quick]
 """
@@ -46,7 +50,7 @@ KINDS = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforc
 # HBM bytes per launch of the search kernels at the plain l = 16 workload, from
 # two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0: the factors of
 # a 256 MiB copy probe); the profile summaries under profiles/ give the commands.
-TRAFFIC_BYTES = {"lower_aggr": 15.55e9, "upper_aggr": 4.73e9, "leaf_rotation": 4.63e9}
+TRAFFIC_BYTES = {"lower_aggr": 15.67e9, "upper_aggr": 4.70e9, "leaf_rotation": 4.69e9}
 
 
 def log(msg: str):
@@ -220,6 +224,26 @@ def batched_4096(gra, ctx, quick: bool, sequential: bool = True) -> dict:
             "bijection_all_sets": bij, "oracle_bytes_sets": sample, "oracle_bytes_equal": oracle_ok}
 
 
+def level_tails(gra, d_keys, leaf: int, bucket: int) -> dict:
+    """One more build of the headline keys (outside the timed steps) on a context
+    created with RS_WAVE_TIMES=1: per kind, the share of the resident search
+    waves' time spent in the search loop, and the time from the median wave's exit
+    to the last exit.  It bounds what running tree levels without a launch
+    boundary between them could recover."""
+    os.environ["RS_WAVE_TIMES"] = "1"
+    try:
+        diag = gra.Context(0)
+    finally:
+        del os.environ["RS_WAVE_TIMES"]
+    rs = gra.GPURecSplit(d_keys, bucket, leaf_size=leaf, ctx=diag)
+    st = rs.stats()
+    rs.close()
+    diag.close()
+    kinds = [k for k in KINDS if st["search_ms"][k] > 0]
+    return {"busy": {k: st["search_busy"][k] for k in kinds}, "tail_ms": {k: st["search_tail_ms"][k] for k in kinds},
+            "search_ms": {k: st["search_ms"][k] for k in kinds}}
+
+
 def served(args) -> dict:
     """Scaled served stream through the construction server: Poisson arrivals
     (seed 4), sizes log-uniform in [1e4, 1e6], l in {8, 12, 16, 24} with weights
@@ -342,6 +366,8 @@ def main() -> int:
         "bijection_verified": head["bijection_verified"],
         "phase_ms": head["phase_ms"],
     }
+    if not args.quick:
+        line["level_tails"] = level_tails(gra, d_keys, args.leaf, args.bucket)
     del d_keys
     ok = head["bijection_verified"]
     d3 = splitmix64_keys_device(2, 5_000_000)
@@ -362,6 +388,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         if args.serve_jobs > 0:
             line["served"] = served(args)
+            half = argparse.Namespace(**vars(args))
+            half.serve_rate = args.serve_rate / 2
+            line["served_half_rate"] = served(half)
     if args.probe:
         line["probe"] = probe(0)
     if not args.no_cpu_baseline and not args.quick:

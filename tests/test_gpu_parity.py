@@ -251,3 +251,26 @@ def test_remap64_bytes_match_oracle(torch_cuda, leaf, bucket, n, rotate):
     assert rs.serialize() == ref.serialize()
     d = torch.from_numpy(keys.view(np.int64)).cuda()
     assert np.array_equal(rs.query_device(d).cpu().numpy().view(np.uint64), ref.query(keys))
+
+
+def test_wave_times_keep_bytes_and_report_shares(torch_cuda, monkeypatch):
+    """RS_WAVE_TIMES only records the search waves' start and exit: the bytes equal
+    the oracle's, and each searched kind reports a busy share in (0, 1] and a
+    non-negative exit tail."""
+    monkeypatch.setenv("RS_WAVE_TIMES", "1")
+    ctx = gra.Context(0)
+    monkeypatch.delenv("RS_WAVE_TIMES")
+    keys = splitmix_keys(4242, 20000)
+    _log("wave times leaf=12 bucket=100 n=20000")
+    rs = gra.GPURecSplit(keys, 100, leaf_size=12, ctx=ctx)
+    assert rs.serialize() == OracleMPHF.build(keys, 12, 100, threads=8).serialize()
+    st = rs.stats()
+    searched = [k for k, ms in st["search_ms"].items() if ms > 0]
+    assert searched
+    for k in searched:
+        assert 0 < st["search_busy"][k] <= 1.0, (k, st["search_busy"][k])
+        assert st["search_tail_ms"][k] >= 0
+    plain = gra.GPURecSplit(keys, 100, leaf_size=12).stats()
+    assert all(v == 0 for v in plain["search_busy"].values())
+    rs.close()
+    ctx.close()
