@@ -408,7 +408,7 @@ static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bo
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
-    } else if (KIND == kLeafRot && a.leaf <= 14) {
+    } else if (KIND == kLeafRot && a.leaf <= 16) {
         k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
     } else {
         k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
@@ -464,6 +464,10 @@ struct SerialOut {
     size_t cap;
     size_t *len;
 };
+
+// The tuning a build of this leaf size uses: the leaf size's own (rs_autotune,
+// rs_ctx_set_tune), else the context's.
+static Tune tune_for(const rs_ctx *c, uint32_t leaf) { return c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune; }
 
 // Idle share of every search launch's tail from the waves' start and exit times
 // (RS_WAVE_TIMES): per level, busy = sum of the waves' times in the search loop /
@@ -639,7 +643,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     uint64_t max_list = 1;
     for (int l = 0; l < kNumLists; l++) max_list = std::max(max_list, cen.count[l]);
     NodeState *state = c->state.get<NodeState>(max_list);
-    const Tune &tn = c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune;
+    const Tune tn = tune_for(c, leaf);
     const uint32_t blocks = (uint32_t)c->num_cus * (uint32_t)tn.blocks_per_cu;
     const uint32_t n_waves = blocks * 4;
     st.search_waves = n_waves;
@@ -1001,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_splitmix_keys(uint64_t seed, uint64_t n
 // the sample is sized so that a build takes about 40 ms.  The result becomes
 // the context's tuning for the leaf size.  Returns a JSON report.
 static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags, uint64_t sample_keys) {
-    const Tune base = c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune;
+    const Tune base = tune_for(c, leaf);
     c->tuned[leaf] = true;  // builds below run with explicit tunings (no recursion)
     const uint64_t n_max = uint64_t(1) << 22;
     uint64_t *keys = c->tune_keys.get<uint64_t>(2 * n_max);
@@ -1016,11 +1020,15 @@ static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags
     };
     uint64_t n = sample_keys ? std::min(sample_keys, n_max) : std::max<uint64_t>(2048, bucket);
     rs_build_stats st0 = trial(base, n);
-    double total0 = 0;
-    for (int k = 0; k < kNumKinds; k++) total0 += st0.search_ms[k];
-    if (!sample_keys) {
-        const double scale = total0 > 0 ? 40.0 / total0 : 64.0;
-        n = std::min<uint64_t>(n_max, (uint64_t)(n * std::min(64.0, std::max(1.0, scale))));
+    auto search_total = [](const rs_build_stats &st) {
+        double t = 0;
+        for (int k = 0; k < kNumKinds; k++) t += st.search_ms[k];
+        return t;
+    };
+    // without a given size, the sample doubles until one search takes >= 40 ms (or
+    // reaches 4M keys): small samples time the launch tails, not the throughput
+    while (!sample_keys && search_total(st0) < 40.0 && n < n_max) {
+        n = std::min<uint64_t>(n_max, 2 * n);
         st0 = trial(base, n);
     }
     const int reps = st0.total_ms > 100 ? 1 : 2;
@@ -1244,7 +1252,7 @@ int rs_ctx_set_tune(rs_ctx *c, uint32_t leaf, const char *tune) {
     if (leaf == 0) {
         c->tune = parse_tune(tune, c->tune);
     } else {
-        c->tune_leaf[leaf] = parse_tune(tune, c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune);
+        c->tune_leaf[leaf] = parse_tune(tune, tune_for(c, leaf));
         c->tuned[leaf] = true;
     }
     return RS_OK;
@@ -1253,7 +1261,7 @@ int rs_ctx_set_tune(rs_ctx *c, uint32_t leaf, const char *tune) {
 int rs_ctx_get_tune(rs_ctx *c, uint32_t leaf, char *buf, size_t cap) {
     if (!c || !buf || !cap || leaf == 1 || leaf > kMaxLeaf) return set_error("rs_ctx_get_tune: bad arguments"), RS_ERR_ARG;
     std::lock_guard<std::mutex> g(c->mu);
-    const std::string t = format_tune(leaf && c->tuned[leaf] ? c->tune_leaf[leaf] : c->tune);
+    const std::string t = format_tune(leaf ? tune_for(c, leaf) : c->tune);
     const size_t k = std::min(cap - 1, t.size());
     memcpy(buf, t.data(), k);
     buf[k] = 0;

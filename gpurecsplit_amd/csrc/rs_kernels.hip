@@ -579,21 +579,22 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
                                                    uint32_t nr, uint64_t u0) {
     const uint64_t S = c_start_seed[kSeedLeaf];
     const uint32_t full = (uint32_t)((1ull << L) - 1);
-    uint32_t ml[U], mr[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) ml[u] = mr[u] = 0;
     const Seeds<U> sd(u0 * L + S, L);
-    if constexpr (R64) {
-        each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { ml[u] |= 1u << r; });
-        each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mr[u] |= 1u << r; });
-    } else {
-        each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
-        each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
-    }
     const uint32_t lane = lane_id();
     if constexpr (!RB) {
-        // Large leaves: few candidates per group (under 4 at L = 16), tested in
-        // index order one at a time across the wave: lane r tests rotation r.
+        // Leaves of 17 to 24 keys: two masks per candidate.  Few candidates per
+        // group are collision-free (1.6 of 256 at L = 18); they are tested in index
+        // order one at a time across the wave: lane r tests rotation r.
+        uint32_t ml[U], mr[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) ml[u] = mr[u] = 0;
+        if constexpr (R64) {
+            each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { ml[u] |= 1u << r; });
+            each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mr[u] |= 1u << r; });
+        } else {
+            each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
+            each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
+        }
         const uint32_t r = lane < L ? lane : 0;
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -610,49 +611,63 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
         }
         return kNone;
     } else {
-    // Small leaves (RB, L <= 14; 13 candidates per group of 256 at L = 12): candidates whose two
-    // masks are collision-free go to a per-wave LDS list in unit order (u, then
-    // lane): left mask | unit << 24 (L <= 24, unit < 256) and the right mask.
-    // Each step then tests floor(64 / L) consecutive candidates: lane i tests
-    // rotation i mod L of candidate i / L, so the lowest set bit of the ballot is
-    // the lowest fitting rotation of the first candidate that has one.
-    __shared__ uint32_t s_cand[4][2][U * 64];  // one list per wave of the workgroup
-    uint32_t *cl = s_cand[threadIdx.x >> 6][0], *cr = s_cand[threadIdx.x >> 6][1];
-    uint32_t count = 0;
+        // Leaves of at most 16 keys (RB): both masks in one word, the left one in
+        // bits 0..L-1 and the right one in bits 16..16+L-1 (the 16 folds into the
+        // remap's multiply-add); both halves are collision-free exactly when L bits
+        // are set.  Many candidates pass (13 of 256 at L = 12, 43 at L = 8): they go
+        // to a per-wave LDS list in unit order (u, then lane), and each step tests
+        // floor(64 / L) of them: lane i tests rotation i mod L of candidate i / L,
+        // so the lowest set bit of the ballot is the lowest fitting rotation of the
+        // first candidate that has one.
+        uint32_t mk[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        const bool ok = (uint32_t)__popc(ml[u]) == nl && (uint32_t)__popc(mr[u]) == nr;
-        const unsigned long long bal = __ballot(ok);
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-        if (ok) {
-            cl[count + below] = ml[u] | ((uint32_t)(u * 64) + lane) << 24;
-            cr[count + below] = mr[u];
+        for (int u = 0; u < U; u++) mk[u] = 0;
+        if constexpr (R64) {
+            each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { mk[u] |= 1u << r; });
+            each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mk[u] |= 1u << (r + 16); });
+        } else {
+            each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, L) >> 16); });
+            each_hash<U, SK>(kp, nl, L, sd,
+                             [&](int u, uint32_t h) { mk[u] |= 1u << ((__umul24(h, L) + (16u << 16)) >> 16); });
         }
-        count += (uint32_t)__popcll(bal);
-    }
-    if (count == 0) return kNone;
-    // the list is read back by other lanes of this wave: wait for the writes
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t per_step = 64 / L;
-    const uint32_t g = (lane * ((65536 + L - 1) / L)) >> 16;  // lane / L (lane < 64, L <= 24)
-    const uint32_t r = lane - g * L;
-    for (uint32_t base = 0; base < count; base += per_step) {
-        const uint32_t idx = base + g;
-        const bool valid = g < per_step && idx < count;
-        const uint32_t el = valid ? cl[idx] : 0;
-        const uint32_t bl = el & 0xFFFFFFu, br = valid ? cr[idx] : 0;
-        const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
-        const unsigned long long fit = __ballot(valid && (bl | rot) == full);
-        if (fit) {
-            const uint32_t first = (uint32_t)__ffsll((long long)fit) - 1;
-            const uint32_t unit = (uint32_t)__builtin_amdgcn_readlane((int)(el >> 24), (int)first);
-            const uint32_t rr = first - (first / L) * L;
-            return (u0 + (uint64_t)unit) * L + (uint64_t)rr;
+        __shared__ uint32_t s_cand[4][2][U * 64];  // one list per wave of the workgroup
+        uint32_t *cm = s_cand[threadIdx.x >> 6][0], *cu = s_cand[threadIdx.x >> 6][1];
+        uint32_t count = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const bool ok = (uint32_t)__popc(mk[u]) == L;
+            const unsigned long long bal = __ballot(ok);
+            const uint32_t below =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (ok) {
+                cm[count + below] = mk[u];
+                cu[count + below] = (uint32_t)(u * 64) + lane;
+            }
+            count += (uint32_t)__popcll(bal);
         }
-    }
-    return kNone;
+        if (count == 0) return kNone;
+        // the list is read back by other lanes of this wave: wait for the writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t per_step = 64 / L;
+        const uint32_t g = (lane * ((65536 + L - 1) / L)) >> 16;  // lane / L (lane < 64, L <= 16)
+        const uint32_t r = lane - g * L;
+        for (uint32_t base = 0; base < count; base += per_step) {
+            const uint32_t idx = base + g;
+            const bool valid = g < per_step && idx < count;
+            const uint32_t e = valid ? cm[idx] : 0;
+            const uint32_t bl = e & 0xFFFFu, br = e >> 16;
+            const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
+            const unsigned long long fit = __ballot(valid && (bl | rot) == full);
+            if (fit) {
+                const uint32_t first = (uint32_t)__ffsll((long long)fit) - 1;
+                const uint32_t gi = first / L;
+                const uint32_t unit = cu[base + gi];
+                return (u0 + (uint64_t)unit) * L + (uint64_t)(first - gi * L);
+            }
+        }
+        return kNone;
     }
 }
 
@@ -694,8 +709,8 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-// RB (rotation leaves of at most 14 keys): the candidate test batched through LDS,
-// whose extra registers are held to five waves per SIMD.
+// RB (rotation leaves of at most 16 keys): packed masks and the candidate test
+// batched through LDS, with registers held to five waves per SIMD.
 template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false>
 __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
     extern __shared__ uint32_t s_lut[];
