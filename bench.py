@@ -20,6 +20,7 @@ and, as extra keys, the other workloads measured on the same GPU:
                       queueing at the server's capacity)
   level_tails         busy share and exit tail of every search launch (one more,
                       untimed build with RS_WAVE_TIMES=1)
+  dropin_host_keys    the headline build from host keys (rs_build: PCIe upload included)
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--quick]
 """
@@ -224,6 +225,23 @@ def batched_4096(gra, ctx, quick: bool, sequential: bool = True) -> dict:
             "bijection_all_sets": bij, "oracle_bytes_sets": sample, "oracle_bytes_equal": oracle_ok}
 
 
+def dropin_host_keys(gra, ctx, d_keys, leaf: int, bucket: int, device_ms: float, steps: int = 2) -> dict:
+    """The drop-in constructor as a C++ caller uses it (rs_build): the keys start in
+    host memory and the build copies them to HBM (PCIe-inclusive).  Not the headline
+    value, which times device-resident keys."""
+    keys = np.ascontiguousarray(d_keys.cpu().numpy().view(np.uint64))
+    gra.GPURecSplit(keys, bucket, leaf_size=leaf, ctx=ctx)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rs = gra.GPURecSplit(keys, bucket, leaf_size=leaf, ctx=ctx)
+        times.append((time.perf_counter() - t0) * 1e3)
+        rs.close()
+    ms = float(np.mean(times))
+    return {"ms_per_step": ms, "value": ms * 1e6 / keys.shape[0], "unit": "ns/object", "steps": steps,
+            "upload_and_host_ms": ms - device_ms, "key_bytes": int(keys.nbytes)}
+
+
 def level_tails(gra, d_keys, leaf: int, bucket: int) -> dict:
     """One more build of the headline keys (outside the timed steps) on a context
     created with RS_WAVE_TIMES=1: per kind, the share of the resident search
@@ -368,6 +386,7 @@ def main() -> int:
     }
     if not args.quick:
         line["level_tails"] = level_tails(gra, d_keys, args.leaf, args.bucket)
+        line["dropin_host_keys"] = dropin_host_keys(gra, ctx, d_keys, args.leaf, args.bucket, head["ms_per_step"])
     del d_keys
     ok = head["bijection_verified"]
     d3 = splitmix64_keys_device(2, 5_000_000)
