@@ -71,7 +71,8 @@ def kernel_rooflines(stats: list) -> dict:
         eps = ev / (ms / 1e3)
         out[k] = {"kernel_ms": ms, "evals": ev, "evals_per_s": eps, "achieved": eps * OPS_PER_EVAL / 1e12,
                   "frac": eps * OPS_PER_EVAL / 1e12 / VALU_PEAK_TOPS, "nodes": last["nodes"][k],
-                  "exec_ratio": last["executed_evals"][k] / ev}
+                  "exec_ratio": last["executed_evals"][k] / ev,
+                  "seeds": last["seeds"][k], "seeds_per_s": last["seeds"][k] / (ms / 1e3)}
     return out
 
 
@@ -102,7 +103,8 @@ def plain(gra, ctx, d_keys, n, leaf, bucket, steps, warmup, label) -> dict:
             "kernels": kernel_rooflines(stats),
             "phase_ms": {"partition": last["partition_ms"], "search": last["search_ms"], "split": last["split_ms"],
                          "encode": last["encode_ms"], "total": last["total_ms"]},
-            "evals_per_s": sum(last["evals"].values()) / (sum(last["search_ms"].values()) / 1e3)}
+            "evals_per_s": sum(last["evals"].values()) / (sum(last["search_ms"].values()) / 1e3),
+            "seeds_tested_per_s": sum(last["seeds"].values()) / (sum(last["search_ms"].values()) / 1e3)}
 
 
 def cpu_baseline(leaf: int, bucket: int, seed: int, threads: int, buckets_per_thread: int = 5) -> dict:
@@ -379,7 +381,8 @@ def main() -> int:
                      "traffic": TRAFFIC_BYTES.get(dom), "ops_per_eval": OPS_PER_EVAL,
                      "evals": kern[dom]["evals"], "kernel_ms": kern[dom]["kernel_ms"],
                      "evals_per_s": kern[dom]["evals_per_s"], "kernels": kern},
-        "seeds_evals_per_s": head["evals_per_s"],
+        "evals_per_s": head["evals_per_s"],
+        "seeds_tested_per_s": head["seeds_tested_per_s"],
         "bits_per_key": head["bits_per_key"],
         "bijection_verified": head["bijection_verified"],
         "phase_ms": head["phase_ms"],

@@ -731,8 +731,8 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     // ---- 5. descriptor lengths (global scan; each set reads its segment)
     uint64_t *bits = c->bits.get<uint64_t>(NB);
     uint64_t *pos = c->pos.get<uint64_t>(NB + 1);
-    unsigned long long *evals = c->evals.get<unsigned long long>(kNumKinds);
-    HIP_CHECK(hipMemsetAsync(evals, 0, kNumKinds * 8, s));
+    unsigned long long *evals = c->evals.get<unsigned long long>(2 * kNumKinds);  // evaluations, then seeds
+    HIP_CHECK(hipMemsetAsync(evals, 0, 2 * kNumKinds * 8, s));
     EncodeArgs ca{acc, node_base, seeds, node_m, memo, memo + kMaxBucket, NB, P, rotate ? 1 : 0};
     k_bucket_bits<<<(uint32_t)std::min<uint64_t>((NB + 3) / 4, (uint64_t)c->num_cus * 8), 256, 0, s>>>(ca, bits, evals);
     HIP_CHECK(hipGetLastError());
@@ -745,11 +745,17 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     std::vector<uint64_t> pos_at(S + 1);
     uint64_t *h64 = (uint64_t *)c->h_pinned;
     HIP_CHECK(hipMemcpyAsync(pos_at.data(), d_gath, (S + 1) * 8, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(h64 + 1, evals, kNumKinds * 8, hipMemcpyDeviceToHost, s));
+    // pinned staging: error flags at byte 64, work counters at 128..296, evaluations
+    // and seeds at 512..592
+    HIP_CHECK(hipMemcpyAsync(h64 + 64, evals, 2 * kNumKinds * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h64 + 16, work, kNumLists * 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(h + 16, err, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    for (int k = 0; k < kNumKinds; k++) st.evals[k] = h64[1 + k];
+    for (int k = 0; k < kNumKinds; k++) {
+        st.evals[k] = h64[64 + k];
+        st.seeds[k] = h64[64 + kNumKinds + k];
+    }
+    static_assert(16 + kNumLists <= 64, "work counters overlap the evaluation counts");
     for (int l = 0; l < kNumLists; l++)
         st.executed_evals[l < (int)kMaxDepth ? kHigher : l - kMaxDepth + 1] += h64[16 + l];
     if (wave_clock) wave_time_stats(c, wave_clock, n_waves, cen.count, st);

@@ -1068,7 +1068,10 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 }
 
 // Per bucket (one wavefront): bits of the bucket's descriptor = fixed bits of the
-// whole subtree (memo) + unary bits of every node; algorithmic evaluations per kind.
+// whole subtree (memo) + unary bits of every node; algorithmic evaluations per kind
+// in evals[0, kNumKinds) and candidate seeds tested per kind in evals[kNumKinds,
+// 2 kNumKinds) (x + 1 per split or brute-force node; (floor(x / L) + 1) L per
+// rotation leaf, whose L rotations of one pass over its keys test L seeds).
 __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__restrict__ bits,
                                                      unsigned long long *__restrict__ evals) {
     // waves stride over the buckets and keep their evaluation counts in registers:
@@ -1076,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__r
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t n_waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint32_t lane = lane_id();
-    uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0};
+    uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0}, sd[kNumKinds] = {0, 0, 0, 0, 0};
     for (uint64_t b = wave; b < a.nb; b += n_waves) {
         const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
         if (m < 2) {
@@ -1093,18 +1096,27 @@ __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__r
             un += (x >> g) + 1;
             const uint32_t kind = classify(a.P, s, a.rotate != 0);
             const uint64_t e = kind == kLeafRot ? (x / s + 1) * s : (x + 1) * s;
+            const uint64_t t = kind == kLeafRot ? e : x + 1;
 #pragma unroll
-            for (int k = 0; k < kNumKinds; k++) ev[k] += (uint32_t)k == kind ? e : 0;
+            for (int k = 0; k < kNumKinds; k++) {
+                ev[k] += (uint32_t)k == kind ? e : 0;
+                sd[k] += (uint32_t)k == kind ? t : 0;
+            }
         }
         un = wave_sum64(un);
         if (lane == 0) bits[b] = (a.memo[m] & 0xFFFF) + un;
     }
 #pragma unroll
-    for (int k = 0; k < kNumKinds; k++) ev[k] = wave_sum64(ev[k]);
+    for (int k = 0; k < kNumKinds; k++) {
+        ev[k] = wave_sum64(ev[k]);
+        sd[k] = wave_sum64(sd[k]);
+    }
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < kNumKinds; k++)
+        for (int k = 0; k < kNumKinds; k++) {
             if (ev[k]) atomicAdd(&evals[k], (unsigned long long)ev[k]);
+            if (sd[k]) atomicAdd(&evals[kNumKinds + k], (unsigned long long)sd[k]);
+        }
     }
 }
 

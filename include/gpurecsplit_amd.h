@@ -94,6 +94,10 @@ typedef struct rs_build_stats {
      * search_tail_ms = time from the median wave's exit to the last exit. */
     double search_busy[8];
     double search_tail_ms[8];
+    /* candidate seeds tested per kind (algorithmic, from the output seeds): x + 1
+     * per split or brute-force node with seed x, (floor(x/L) + 1) * L per rotation
+     * leaf (each pass over its L keys tests L rotations, i.e. L seeds) */
+    uint64_t seeds[8];
 } rs_build_stats;
 
 /* Context bound to one GPU (device ordinal as seen by HIP). */

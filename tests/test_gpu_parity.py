@@ -274,3 +274,53 @@ def test_wave_times_keep_bytes_and_report_shares(torch_cuda, monkeypatch):
     assert all(v == 0 for v in plain["search_busy"].values())
     rs.close()
     ctx.close()
+
+
+def _tree_sizes(m, leaf):
+    """Node sizes of a bucket's split tree in DFS preorder (children of size 1 carry
+    no seed), by the split rules of AbstractParallelRecSplit.hpp:133-148 and the
+    fanout-2 split size of SIMDRecSplit.hpp:632."""
+    lower = leaf * max(2, -(-(35 * leaf + 50) // 100))
+    upper = lower * -(-(21 * leaf + 90) // 100)
+    out, stack = [], [m]
+    while stack:
+        s = stack.pop()
+        out.append(s)
+        if s <= leaf:
+            continue
+        if s > upper:
+            split = ((s // 2 + upper - 1) // upper) * upper
+            kids = [split, s - split]
+        else:
+            unit = lower if s > lower else leaf
+            fan = -(-s // unit)
+            kids = [unit] * (fan - 1) + [s - unit * (fan - 1)]
+        stack.extend(k for k in reversed(kids) if k > 1)
+    return out
+
+
+def test_seed_counts_from_the_seeds(torch_cuda):
+    """rs_build_stats.seeds: x + 1 per split or brute-force node, (x // L + 1) * L per
+    rotation leaf, recomputed from the structure's own seeds (decoded by the oracle)
+    and node sizes (a Python tree walk over the bucket sizes)."""
+    leaf, bucket, n = 12, 100, 20000
+    keys = splitmix_keys(77, n)
+    rs = gra.GPURecSplit(keys, bucket, leaf_size=leaf)
+    st = rs.stats()
+    seeds = OracleMPHF.load(rs.serialize(), leaf).seeds()
+    nb = -(-n // bucket)
+    first = keys[:, 0]
+    hi, lo = first >> np.uint64(32), first & np.uint64(0xFFFFFFFF)
+    b = (hi * np.uint64(nb) + ((lo * np.uint64(nb)) >> np.uint64(32))) >> np.uint64(32)
+    sizes = np.bincount(b.astype(np.int64), minlength=nb)
+    expect, i = 0, 0
+    for m in sizes:
+        if m <= 1:
+            continue
+        for s in _tree_sizes(int(m), leaf):
+            x = int(seeds[i])
+            i += 1
+            expect += (x // leaf + 1) * leaf if s == leaf else x + 1
+    assert i == len(seeds)
+    assert sum(st["seeds"].values()) == expect
+    assert st["seeds"]["leaf_rotation"] == st["evals"]["leaf_rotation"]
