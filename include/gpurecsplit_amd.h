@@ -123,8 +123,10 @@ int rs_build_device(rs_ctx *ctx, const uint64_t *d_keys, size_t n, uint32_t leaf
                     size_t bucket_size, int rotate, rs_mphf **out);
 
 /* Allocation-free construction: the same build as rs_build_device, for callers that
- * cannot allocate device memory (code under stream capture, tight serving loops,
- * harnesses that forbid hipMalloc).  No reference counterpart.  Every device
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  The call synchronises the library's own
+ * stream between its phases (it reads sizes back to the host), so it cannot be
+ * recorded by a caller's stream capture.  Every device
  * buffer of the build comes from a fixed 64 MiB scratch array that the library
  * holds per GPU until the process ends; the call makes no hipMalloc, hipFree or
  * pinned/torch allocation.  Inputs bounded by 75*n + 89*ceil(n/bucket_size) +
