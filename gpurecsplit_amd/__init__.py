@@ -23,7 +23,7 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["GPURecSplit", "RecSplitError", "Context", "build_batch", "build_noalloc", "golomb_memo", "hash_strings",
+__all__ = ["GPURecSplit", "RecSplitError", "Context", "Plan", "build_batch", "build_noalloc", "golomb_memo", "hash_strings",
            "LIB_PATH", "STATUS"]
 
 LIB_PATH = Path(__file__).resolve().parent / "librecsplit_amd.so"
@@ -124,6 +124,11 @@ def _load():
         "rs_build_strings": (C.c_int, [vp, C.c_char_p, u64p, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int,
                                        C.POINTER(vp)]),
         "rs_mphf_query_string": (C.c_uint64, [vp, C.c_char_p, C.c_size_t]),
+        "rs_plan_create": (C.c_int, [vp, C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.POINTER(vp)]),
+        "rs_plan_enqueue": (C.c_int, [vp, C.c_void_p, C.c_void_p]),
+        "rs_plan_result": (C.c_int, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "rs_plan_device_bytes": (C.c_size_t, [vp]),
+        "rs_plan_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -138,12 +143,12 @@ lib = _load()
 # then structures, then contexts) by an exit hook that runs before interpreter
 # teardown -- while the HIP runtime and torch are still fully alive -- instead of
 # by __del__ calls in whatever order module teardown reaches them.
-_live = {"server": weakref.WeakSet(), "mphf": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+_live = {"server": weakref.WeakSet(), "plan": weakref.WeakSet(), "mphf": weakref.WeakSet(), "ctx": weakref.WeakSet()}
 
 
 @atexit.register
 def _release_all():
-    for kind in ("server", "mphf", "ctx"):
+    for kind in ("server", "plan", "mphf", "ctx"):
         for obj in list(_live[kind]):
             obj.close()
 
@@ -391,6 +396,67 @@ def build_batch(d_keys, set_offsets, bucket_size: int, leaf_size: int, rotate: b
     out = [GPURecSplit(bucket_size=bucket_size, leaf_size=leaf_size, rotate=rotate, ctx=ctx, _handle=C.c_void_p(h),
                        remap64=remap64) for h in handles]
     return out, st.as_dict()
+
+
+class Plan:
+    """A build plan (rs_plan_*): every device buffer of the construction of one
+    MPHF of n keys, allocated once; `enqueue` queues a whole build on a stream with
+    no allocation and no host synchronisation, so it can be recorded into a CUDA/HIP
+    graph (torch.cuda.graph) and replayed for new keys in the same device buffer.
+
+        plan = Plan(n, bucket_size=100, leaf_size=8)
+        plan.enqueue(d_keys)          # current stream; also inside torch.cuda.graph
+        torch.cuda.synchronize()
+        data = plan.result()          # serialized bytes == GPURecSplit(d_keys, ...).serialize()
+    """
+
+    def __init__(self, n: int, bucket_size: int, leaf_size: int, rotate: bool = True, remap64: bool = False,
+                 ctx: Context | None = None):
+        self._ctx = ctx or Context.default()
+        self.n, self.bucket_size, self.leaf_size = n, bucket_size, leaf_size
+        self.rotate, self.remap64 = rotate, remap64
+        h = C.c_void_p()
+        _check(lib.rs_plan_create(self._ctx.h, n, leaf_size, bucket_size, _flags(rotate, remap64), C.byref(h)))
+        self.h = h
+        _live["plan"].add(self)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib.rs_plan_device_bytes(self.h))
+
+    def enqueue(self, d_keys, stream=None):
+        """Queues a build of d_keys (a contiguous (n, 2) 64-bit CUDA tensor) on
+        `stream` (a torch.cuda.Stream; None: torch's current stream)."""
+        if d_keys.dim() != 2 or d_keys.shape[0] != self.n or d_keys.shape[1] != 2 or d_keys.element_size() != 8 \
+                or not d_keys.is_contiguous():
+            raise ValueError(f"d_keys must be a contiguous ({self.n}, 2) 64-bit CUDA tensor")
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream(d_keys.device)
+        _check(lib.rs_plan_enqueue(self.h, C.c_void_p(d_keys.data_ptr()), C.c_void_p(st.cuda_stream)))
+
+    def result(self) -> bytes:
+        """The serialized structure of the last build (after its stream's work is
+        done); raises RecSplitError for a failed build."""
+        n = C.c_size_t()
+        _check(lib.rs_plan_result(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _check(lib.rs_plan_result(self.h, buf, n.value, C.byref(n)))
+        return buf.raw[: n.value]
+
+    def build(self, d_keys) -> bytes:
+        """enqueue on the current stream, wait, and return result()."""
+        import torch
+        self.enqueue(d_keys)
+        torch.cuda.current_stream(d_keys.device).synchronize()
+        return self.result()
+
+    def close(self, _destroy=lib.rs_plan_destroy):
+        if getattr(self, "h", None):
+            _destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def build_noalloc(d_keys, bucket_size: int, leaf_size: int, rotate: bool = True, ctx: Context | None = None,

@@ -21,6 +21,18 @@ constexpr uint32_t kMaxDepth = 17;   // fanout-2 levels use start_seed[0..16]
 // search so that each wave reaches its exit even on adversarial input.
 constexpr uint64_t kSeedLimit = uint64_t(1) << 44;
 
+// Bits of a build's device error word.
+constexpr uint32_t kErrDuplicateKeys = 1;   // two keys of one bucket share `second`
+constexpr uint32_t kErrSeedLimit = 2;       // a search passed kSeedLimit
+constexpr uint32_t kErrEfJump = 4;          // an Elias-Fano jump offset exceeds 16 bits
+constexpr uint32_t kErrTimeout = 8;         // the context's time limit passed
+constexpr uint32_t kErrBucketTooLarge = 16; // a bucket of kMaxBucket or more keys (build plans)
+constexpr uint32_t kErrRiceCap = 32;        // descriptors exceed a plan's Golomb-Rice bound
+constexpr uint32_t kErrEfCap = 64;          // Elias-Fano arrays exceed a plan's bounds
+// Errors after which a plan's search and encoding kernels write nothing.
+constexpr uint32_t kErrStopSearch = kErrDuplicateKeys | kErrBucketTooLarge;
+constexpr uint32_t kErrStopEncode = kErrStopSearch | kErrRiceCap | kErrEfCap;
+
 // Starting seed of each tree level (AbstractParallelRecSplit.hpp:95-98): fanout-2
 // nodes use [depth], upper aggregation [17], lower aggregation [18], leaves [19].
 #define RS_START_SEEDS                                                                          \

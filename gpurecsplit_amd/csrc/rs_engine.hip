@@ -327,24 +327,24 @@ static uint64_t scan_tmp_words(uint64_t n) {
     return words + 1;
 }
 
-static void scan_level(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out, uint64_t *tmp) {
+static void scan_level(hipStream_t s, const void *in, bool in32, uint64_t n, uint64_t *out, uint64_t *tmp) {
     const uint64_t items = n + 1;
     const uint64_t nblk = (items + kScanItems - 1) / kScanItems;
     uint64_t *sums = tmp, *offs = tmp + nblk;  // nblk block sums, then their scan (nblk + 1)
     if (in32)
-        k_scan_blocks<uint32_t><<<nblk, 256, 0, c->stream>>>((const uint32_t *)in, n, out, sums);
+        k_scan_blocks<uint32_t><<<nblk, 256, 0, s>>>((const uint32_t *)in, n, out, sums);
     else
-        k_scan_blocks<uint64_t><<<nblk, 256, 0, c->stream>>>((const uint64_t *)in, n, out, sums);
+        k_scan_blocks<uint64_t><<<nblk, 256, 0, s>>>((const uint64_t *)in, n, out, sums);
     HIP_CHECK(hipGetLastError());
     if (nblk > 1) {
-        scan_level(c, sums, false, nblk, offs, tmp + 2 * nblk + 1);
-        k_scan_add<<<(items + 255) / 256, 256, 0, c->stream>>>(out, n, offs);
+        scan_level(s, sums, false, nblk, offs, tmp + 2 * nblk + 1);
+        k_scan_add<<<(items + 255) / 256, 256, 0, s>>>(out, n, offs);
         HIP_CHECK(hipGetLastError());
     }
 }
 
 static void scan_u64(rs_ctx *c, const void *in, bool in32, uint64_t n, uint64_t *out) {
-    scan_level(c, in, in32, n, out, c->scan_tmp.get<uint64_t>(scan_tmp_words(n)));
+    scan_level(c->stream, in, in32, n, out, c->scan_tmp.get<uint64_t>(scan_tmp_words(n)));
 }
 
 // Device tables of a leaf size: the Golomb-Rice memo, then the unmasked node
@@ -400,9 +400,8 @@ static double ev_ms(rs_ctx *c, int a, int b) {
 }
 
 template <int KIND, int U, bool SK, bool R64>
-static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
+static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
                         size_t lds) {
-    hipStream_t s = c->stream;
     if (KIND == kUpper || KIND == kLower) {
         if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
@@ -416,34 +415,34 @@ static void launch_kind(rs_ctx *c, const uint64_t *keys, const SearchArgs &a, bo
 }
 
 template <int U, bool SK, bool R64>
-static void launch_u(rs_ctx *c, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
+static void launch_u(hipStream_t s, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
                      uint32_t blocks, size_t lds) {
     switch (kind) {
-    case kHigher: launch_kind<kHigher, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
-    case kUpper: launch_kind<kUpper, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLower: launch_kind<kLower, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
-    case kLeafRot: launch_kind<kLeafRot, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
-    default: launch_kind<kLeafBrute, U, SK, R64>(c, keys, a, wide, lut, blocks, lds); break;
+    case kHigher: launch_kind<kHigher, U, SK, R64>(s, keys, a, wide, lut, blocks, lds); break;
+    case kUpper: launch_kind<kUpper, U, SK, R64>(s, keys, a, wide, lut, blocks, lds); break;
+    case kLower: launch_kind<kLower, U, SK, R64>(s, keys, a, wide, lut, blocks, lds); break;
+    case kLeafRot: launch_kind<kLeafRot, U, SK, R64>(s, keys, a, wide, lut, blocks, lds); break;
+    default: launch_kind<kLeafBrute, U, SK, R64>(s, keys, a, wide, lut, blocks, lds); break;
     }
 }
 
 template <bool SK, bool R64>
-static void launch_sk(rs_ctx *c, int u, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut,
-                      uint32_t blocks, size_t lds) {
+static void launch_sk(hipStream_t s, int u, uint32_t kind, const uint64_t *keys, const SearchArgs &a, bool wide,
+                      bool lut, uint32_t blocks, size_t lds) {
     switch (u) {
-    case 1: launch_u<1, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    case 4: launch_u<4, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
-    default: launch_u<2, SK, R64>(c, kind, keys, a, wide, lut, blocks, lds); break;
+    case 1: launch_u<1, SK, R64>(s, kind, keys, a, wide, lut, blocks, lds); break;
+    case 4: launch_u<4, SK, R64>(s, kind, keys, a, wide, lut, blocks, lds); break;
+    default: launch_u<2, SK, R64>(s, kind, keys, a, wide, lut, blocks, lds); break;
     }
 }
 
 // The 64-bit remap runs with keys in scalar registers only.
-static void launch_search(rs_ctx *c, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
+static void launch_search(hipStream_t s, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
                           bool wide, bool lut, uint32_t blocks, size_t lds, bool r64) {
     const int u = tn.u[kind];
-    if (r64) launch_sk<true, true>(c, u, kind, keys, a, wide, lut, blocks, lds);
-    else if (tn.sk[kind]) launch_sk<true, false>(c, u, kind, keys, a, wide, lut, blocks, lds);
-    else launch_sk<false, false>(c, u, kind, keys, a, wide, lut, blocks, lds);
+    if (r64) launch_sk<true, true>(s, u, kind, keys, a, wide, lut, blocks, lds);
+    else if (tn.sk[kind]) launch_sk<true, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
+    else launch_sk<false, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -688,15 +687,8 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.groups = (uint32_t)tn.groups[kind];
         a.help_cap = (uint32_t)tn.help_cap[kind];
         // Levels of many small nodes (millions of leaves of many small sets) would
-        // otherwise be bound by the one fresh-node counter (one atomic per node):
-        // tickets take up to 16 nodes, except for the last 8 per wave.
-        a.claim_batch = 1;
-        a.batch_tickets = 0;
-        const uint64_t tail_nodes = (uint64_t)n_waves * 8;
-        if (cnt > tail_nodes) {
-            a.claim_batch = (uint32_t)std::min<uint64_t>(16, std::max<uint64_t>(1, cnt / ((uint64_t)n_waves * 64)));
-            if (a.claim_batch > 1) a.batch_tickets = (uint32_t)((cnt - tail_nodes) / a.claim_batch);
-        }
+        // otherwise be bound by the one fresh-node counter (one atomic per node).
+        claim_params(cnt, n_waves, a.claim_batch, a.batch_tickets);
         bool wide = false, lut = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
@@ -714,7 +706,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             if (!lut) lds = 0;
         }
         PhaseEvents pe{kind, ev_record(c), -1, -1};
-        launch_search(c, tn, kind, sorted, a, wide, lut, blocks, lds, r64);
+        launch_search(s, tn, kind, sorted, a, wide, lut, blocks, lds, r64);
         pe.search1 = ev_record(c);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);
@@ -1435,3 +1427,5 @@ int rs_query_device(rs_ctx *c, const rs_mphf *h, const uint64_t *d_keys, size_t 
 }
 
 }  // extern "C"
+
+#include "rs_plan.hip"

@@ -353,7 +353,31 @@ struct SearchArgs {
     // Optional (nullptr: off): [2 * wave] start and [2 * wave + 1] exit of every
     // wave on the device wall clock, for the idle share of the launch's tail.
     unsigned long long *wave_clock;
+    // Device-driven launch (build plans, nullptr otherwise): the level's node count
+    // is count_dev[list] and its nodes start at nodes + lbase_dev[list]; n_nodes,
+    // claim_batch and batch_tickets are then derived in the kernel.
+    const uint64_t *count_dev;
+    const uint64_t *lbase_dev;
+    uint32_t list;
+    // Optional: the launch does nothing when *stop has a bit of stop_mask set
+    // (duplicate keys or an oversized bucket found by the partition).
+    const uint32_t *stop;
+    uint32_t stop_mask;
 };
+
+// Node claim parameters of a level of n_nodes nodes for n_waves waves: tickets take
+// up to 16 nodes, except for the last 8 per wave (one node per ticket).
+__host__ __device__ inline void claim_params(uint32_t n_nodes, uint32_t n_waves, uint32_t &claim_batch,
+                                             uint32_t &batch_tickets) {
+    claim_batch = 1;
+    batch_tickets = 0;
+    const uint64_t tail_nodes = (uint64_t)n_waves * 8;
+    if (n_nodes > tail_nodes) {
+        const uint64_t b = (uint64_t)n_nodes / ((uint64_t)n_waves * 64);
+        claim_batch = (uint32_t)(b < 1 ? 1 : (b > 16 ? 16 : b));
+        if (claim_batch > 1) batch_tickets = (uint32_t)((n_nodes - tail_nodes) / claim_batch);
+    }
+}
 
 // Build start on the device wall clock (100 MHz constant clock), the reference
 // point of the time limit.
@@ -717,6 +741,14 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (a.wave_clock && lane == 0) a.wave_clock[2 * wave] = (unsigned long long)wall_clock64();
+    if (a.stop && (uni(*a.stop) & a.stop_mask)) return;
+    uint32_t n_nodes = a.n_nodes, claim_batch = a.claim_batch, batch_tickets = a.batch_tickets;
+    const NodeDesc *nodes = a.nodes;
+    if (a.count_dev) {
+        n_nodes = uni((uint32_t)a.count_dev[a.list]);
+        nodes = a.nodes + uni64(a.lbase_dev[a.list]);
+        claim_params(n_nodes, a.n_waves, claim_batch, batch_tickets);
+    }
     const uint64_t units_per_chunk = (uint64_t)a.groups * U * 64;
     const uint64_t scale = KIND == kLeafRot ? a.leaf : 1;
     // Aggregation increment table, one per workgroup: entry r (a remapped value)
@@ -758,13 +790,13 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
                     uint32_t t = 0;
                     if (lane == 0) t = atomicAdd(&a.ctl[0], 1u);
                     t = uni(t);
-                    if (t < a.batch_tickets) {
-                        batch_next = t * a.claim_batch;
-                        batch_end = batch_next + a.claim_batch;
+                    if (t < batch_tickets) {
+                        batch_next = t * claim_batch;
+                        batch_end = batch_next + claim_batch;
                     } else {
-                        const uint32_t c = a.batch_tickets * a.claim_batch + (t - a.batch_tickets);
+                        const uint32_t c = batch_tickets * claim_batch + (t - batch_tickets);
                         batch_next = c;
-                        batch_end = c < a.n_nodes ? c + 1 : c;
+                        batch_end = c < n_nodes ? c + 1 : c;
                     }
                 }
                 if (batch_next < batch_end) {
@@ -827,7 +859,7 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
                 __hip_atomic_store(&a.running[wave], node, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 atomicAdd(&a.state[node].waves, 1u);
             }
-            const NodeDesc d = a.nodes[node];
+            const NodeDesc d = nodes[node];
             key_off = uni64(d.key_off);
             m = uni(d.m);
             {
@@ -957,14 +989,21 @@ __global__ __launch_bounds__(256) void k_state_init(NodeState *st, uint32_t n) {
     }
 }
 
-// Leaves: copy each node's final seed to its slot.
+// Leaves: copy each node's final seed to its slot.  With count_dev (build plans)
+// the level's size and first node come from the device, grid-stride.
 __global__ __launch_bounds__(256) void k_finalize(const NodeDesc *__restrict__ nodes, const NodeState *__restrict__ st,
-                                                  uint32_t n, uint64_t *__restrict__ seeds, uint32_t *__restrict__ err) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t x = st[i].result;
-    seeds[nodes[i].slot] = x;
-    if (x >= kSeedLimit) atomicOr(err, 2u);
+                                                  uint32_t n, uint64_t *__restrict__ seeds, uint32_t *__restrict__ err,
+                                                  const uint64_t *__restrict__ count_dev = nullptr,
+                                                  const uint64_t *__restrict__ lbase_dev = nullptr, uint32_t list = 0) {
+    if (count_dev) {
+        n = (uint32_t)count_dev[list];
+        nodes += lbase_dev[list];
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t x = st[i].result;
+        seeds[nodes[i].slot] = x;
+        if (x >= kSeedLimit) atomicOr(err, 2u);
+    }
 }
 
 // Split nodes: record the seed and partition the node's keys in place by child,
@@ -972,11 +1011,10 @@ __global__ __launch_bounds__(256) void k_finalize(const NodeDesc *__restrict__ n
 // even `second` come first (stable within each parity): full leaves then split
 // into their left and right rotation sets by position.  Predicates are order-free,
 // so this layout does not change any seed.
-__global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
-                                              const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
-                                              uint32_t *__restrict__ err, SplitParams P, uint32_t kind, int r64) {
-    extern __shared__ uint64_t s_keys[];
-    const uint32_t node = blockIdx.x;
+__device__ __forceinline__ void split_node(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
+                                           const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
+                                           uint32_t *__restrict__ err, SplitParams P, uint32_t kind, int r64,
+                                           uint32_t node, uint64_t *s_keys) {
     const uint32_t lane = threadIdx.x;
     const NodeDesc d = nodes[node];
     const uint64_t x = st[node].result;
@@ -1047,6 +1085,26 @@ __global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const
     }
 }
 
+__global__ __launch_bounds__(64) void k_split(uint64_t *__restrict__ keys, const NodeDesc *__restrict__ nodes,
+                                              const NodeState *__restrict__ st, uint64_t *__restrict__ seeds,
+                                              uint32_t *__restrict__ err, SplitParams P, uint32_t kind, int r64,
+                                              const uint64_t *__restrict__ count_dev = nullptr,
+                                              const uint64_t *__restrict__ lbase_dev = nullptr, uint32_t list = 0) {
+    extern __shared__ uint64_t s_keys[];
+    // one node per workgroup; with count_dev (build plans) the level's size and
+    // first node come from the device and workgroups stride over the nodes
+    uint32_t n_nodes = gridDim.x;
+    if (count_dev) {
+        n_nodes = (uint32_t)count_dev[list];
+        nodes += lbase_dev[list];
+    }
+    for (uint32_t node = blockIdx.x; node < n_nodes; node += gridDim.x) {
+        split_node(keys, nodes, st, seeds, err, P, kind, r64, node, s_keys);
+        __syncthreads();  // s_keys is reused by the next node
+    }
+}
+
+
 // ------------------------------------------------------------------ encoding
 
 struct EncodeArgs {
@@ -1059,6 +1117,9 @@ struct EncodeArgs {
     uint64_t nb;
     SplitParams P;
     int rotate;
+    // Optional (build plans): the encoding kernels write nothing when *stop has a
+    // bit of kErrStopEncode set (a size bound or the partition failed).
+    const uint32_t *stop;
 };
 
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
@@ -1082,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_bucket_bits(EncodeArgs a, uint64_t *__r
     uint64_t ev[kNumKinds] = {0, 0, 0, 0, 0}, sd[kNumKinds] = {0, 0, 0, 0, 0};
     for (uint64_t b = wave; b < a.nb; b += n_waves) {
         const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
-        if (m < 2) {
+        if (m < 2 || m >= kMaxBucket) {  // oversized buckets fail the build; no table entry
             if (lane == 0) bits[b] = 0;
             continue;
         }
@@ -1147,8 +1208,9 @@ __global__ __launch_bounds__(256) void k_rice_write(EncodeArgs a, const uint64_t
     const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t lane = lane_id();
     if (b >= a.nb) return;
+    if (a.stop && (*a.stop & kErrStopEncode)) return;
     const uint32_t m = (uint32_t)(a.acc[b + 1] - a.acc[b]);
-    if (m < 2) return;
+    if (m < 2 || m >= kMaxBucket) return;
     const uint64_t base = a.node_base[b];
     const uint32_t nn = a.tree_nodes[m];
     const uint64_t p0 = pos[b] + (uint64_t)bit_off[b];
@@ -1191,7 +1253,9 @@ struct SetEnc {
 // (appendFixed(1, 1), SIMDRecSplit.hpp:810).
 __global__ __launch_bounds__(256) void k_set_offsets(const SetEnc *__restrict__ sets, const uint64_t *__restrict__ pos,
                                                      long long *__restrict__ bit_off,
-                                                     unsigned long long *__restrict__ words) {
+                                                     unsigned long long *__restrict__ words,
+                                                     const uint32_t *__restrict__ stop = nullptr) {
+    if (stop && (*stop & kErrStopEncode)) return;
     const SetEnc e = sets[blockIdx.x];
     const long long off = (long long)(e.rice_word * 64) - (long long)pos[e.bbase];
     for (uint64_t i = threadIdx.x; i < e.nb; i += blockDim.x) bit_off[e.bbase + i] = off;
@@ -1256,7 +1320,8 @@ __global__ __launch_bounds__(256) void k_ef_fill(const SetEnc *__restrict__ sets
                                                  const uint64_t *__restrict__ acc, const uint64_t *__restrict__ pos,
                                                  unsigned long long *__restrict__ lower, unsigned long long *__restrict__ upper_c,
                                                  unsigned long long *__restrict__ upper_p, uint8_t *__restrict__ jump,
-                                                 uint32_t *__restrict__ err) {
+                                                 uint32_t *__restrict__ err, const uint32_t *__restrict__ stop = nullptr) {
+    if (stop && (*stop & kErrStopEncode)) return;
     const SetEnc e = sets[blockIdx.x];
     const EfScalars f = sc[blockIdx.x];
     const uint64_t *cum = acc + e.bbase, *ps = pos + e.bbase;

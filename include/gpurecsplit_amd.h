@@ -172,6 +172,35 @@ uint64_t rs_ctx_device_bytes(rs_ctx *ctx);
 int rs_ctx_release(rs_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
+ * Build plans: the construction of one MPHF of a fixed key count, queued on the
+ * caller's HIP stream with no device allocation and no host synchronisation, so
+ * that it can be recorded by stream capture into a hipGraph and replayed for new
+ * keys.  No reference counterpart (the reference constructor is synchronous,
+ * GPURecSplit.cuh:484-489).
+ *
+ * rs_plan_create sizes every device buffer for n keys (one allocation, about
+ * 90 bytes per key; rs_plan_device_bytes) and uploads the leaf size's tables.
+ * rs_plan_enqueue queues a build of the n keys (first, second pairs) at d_keys, in
+ * device memory of the plan's GPU, on `stream` (a hipStream_t; NULL: the null
+ * stream); it only launches kernels, and the keys must stay valid until the
+ * stream's work is done.  After the caller has synchronised the stream,
+ * rs_plan_result copies the serialized structure (rs_mphf_serialize's layout,
+ * byte-identical to rs_build_device's) into out[0, *len) (out == NULL: only *len)
+ * and reports the build's errors (duplicate keys, a bucket of 3000 keys, ...; a
+ * structure beyond the plan's generous size bounds fails with RS_ERR_ARG).
+ * Enqueues of one plan share its buffers: a second enqueue must not start before
+ * the first one's result has been taken; use one plan per concurrent stream.
+ * Destroy plans before their context.
+ * ------------------------------------------------------------------------- */
+typedef struct rs_plan rs_plan;
+
+int rs_plan_create(rs_ctx *ctx, size_t n, uint32_t leaf_size, size_t bucket_size, int rotate, rs_plan **out);
+int rs_plan_enqueue(rs_plan *plan, const uint64_t *d_keys, void *stream);
+int rs_plan_result(rs_plan *plan, uint8_t *out, size_t cap, size_t *len);
+size_t rs_plan_device_bytes(const rs_plan *plan);
+void rs_plan_destroy(rs_plan *plan);
+
+/* ---------------------------------------------------------------------------
  * Construction server: asynchronous jobs with continuous batching on one GPU.
  * No reference counterpart (the reference builds one MPHF per synchronous
  * constructor call, GPURecSplit.cuh:484-489).
