@@ -21,6 +21,7 @@ and, as extra keys, the other workloads measured on the same GPU:
   level_tails         busy share and exit tail of every search launch (one more,
                       untimed build with RS_WAVE_TIMES=1)
   dropin_host_keys    the headline build from host keys (rs_build: PCIe upload included)
+  plan_graph          small builds: ordinary build vs build plan vs captured-graph replay
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--quick]
 """
@@ -244,6 +245,60 @@ def dropin_host_keys(gra, ctx, d_keys, leaf: int, bucket: int, device_ms: float,
             "upload_and_host_ms": ms - device_ms, "key_bytes": int(keys.nbytes)}
 
 
+def plan_graph(gra, ctx, splitmix64_keys_device, reps: int = 20) -> dict:
+    """Small builds three ways, median wall ms of `reps` each with the serialized
+    bytes in host memory at the end: the ordinary build (the host reads sizes back
+    between phases), a build plan (rs_plan_enqueue: one queued build, one
+    synchronisation) and a replay of the plan captured in a graph.  All three
+    give the same bytes."""
+    import torch
+    out = {}
+    for n, leaf, b in [(10_000, 8, 100), (100_000, 12, 100), (100_000, 16, 2000), (1_000_000, 8, 100)]:
+        keys = splitmix64_keys_device(20 + leaf, n)
+        torch.cuda.synchronize()
+
+        def timed(fn):
+            ts, data = [], None
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                data = fn()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            return float(np.median(ts)), data
+
+        def eager():
+            rs = gra.GPURecSplit(keys, b, leaf_size=leaf, ctx=ctx)
+            d = rs.serialize()
+            rs.close()
+            return d
+
+        eager()
+        e_ms, ref = timed(eager)
+        plan = gra.Plan(n, b, leaf, ctx=ctx)
+        plan.build(keys)
+        p_ms, p_data = timed(lambda: plan.build(keys))
+        static = keys.clone()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            plan.enqueue(static)
+
+        def replay():
+            g.replay()
+            torch.cuda.synchronize()
+            return plan.result()
+
+        replay()
+        g_ms, g_data = timed(replay)
+        out[f"n{n}_l{leaf}_b{b}"] = {"build_ms": e_ms, "plan_ms": p_ms, "graph_replay_ms": g_ms,
+                                     "same_bytes": p_data == ref and g_data == ref,
+                                     "plan_device_bytes": plan.device_bytes}
+        log(f"plan/graph n={n} l={leaf} b={b}: build {e_ms:.2f} ms, plan {p_ms:.2f} ms, graph {g_ms:.2f} ms")
+        del g
+        plan.close()
+    return out
+
+
 def level_tails(gra, d_keys, leaf: int, bucket: int) -> dict:
     """One more build of the headline keys (outside the timed steps) on a context
     created with RS_WAVE_TIMES=1: per kind, the share of the resident search
@@ -323,7 +378,7 @@ def main() -> int:
     ap.add_argument("--no-4096", action="store_true", help="skip the 4096-set batched workload")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched4096: skip the 4096 one-by-one builds (profiling runs)")
-    ap.add_argument("--only", choices=["l18", "l24", "batched4x5M", "batched4096", "served"],
+    ap.add_argument("--only", choices=["l18", "l24", "batched4x5M", "batched4096", "served", "plan_graph"],
                     help="run one extra workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -350,6 +405,8 @@ def main() -> int:
             out = batched_4x5m(gra, ctx, args, float("nan"))
         elif args.only == "batched4096":
             out = batched_4096(gra, ctx, quick=False, sequential=not args.no_sequential)
+        elif args.only == "plan_graph":
+            out = plan_graph(gra, ctx, splitmix64_keys_device)
         else:
             out = served(args)
         print(json.dumps({"only": args.only, **out}))
@@ -404,6 +461,7 @@ def main() -> int:
         line["plain_l24_scaled"] = l24
         ok &= l24["bijection_verified"]
         del d24
+        line["plan_graph"] = plan_graph(gra, ctx, splitmix64_keys_device)
         line["batched_4x5M"] = batched_4x5m(gra, ctx, args, head["ms_per_step"])
         if not args.no_4096:
             line["batched_4096"] = batched_4096(gra, ctx, quick=False)

@@ -180,12 +180,13 @@ __global__ __launch_bounds__(256) void k_scatter(const KeyTile *__restrict__ til
 // (canonical order, independent of the scatter's atomics) and detection of equal
 // values, which no seed can ever separate.
 __global__ __launch_bounds__(256) void k_sort_buckets(uint64_t *__restrict__ sorted, const uint64_t *__restrict__ acc,
-                                                      uint32_t *__restrict__ err, uint32_t rot_leaf) {
-    extern __shared__ uint64_t s[];  // next power of two of the largest bucket
+                                                      uint32_t *__restrict__ err, uint32_t rot_leaf,
+                                                      uint32_t min_m = 0, uint32_t max_m = kMaxBucket - 1) {
+    extern __shared__ uint64_t s[];  // next power of two of max_m (the largest bucket it sorts)
     const uint64_t b = blockIdx.x;
     const uint64_t off = acc[b];
     const uint32_t m = (uint32_t)(acc[b + 1] - off);
-    if (m < 2 || m >= kMaxBucket) return;
+    if (m < 2 || m >= kMaxBucket || m < min_m || m > max_m) return;
     uint32_t P = 1;
     while (P < m) P <<= 1;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) s[i] = i < m ? sorted[off + i] : ~0ull;

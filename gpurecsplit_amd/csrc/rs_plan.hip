@@ -209,7 +209,9 @@ struct rs_plan {
     size_t mem_bytes = 0;
     Tune tune;
     uint32_t blocks = 0, n_waves = 0, n_tiles = 0;
-    std::vector<int> lists;  // tree levels some bucket size can have
+    std::vector<int> lists;     // tree levels some bucket size can have
+    std::vector<char> likely;   // ... and whether a bucket of the likely sizes has it
+    uint32_t likely_max_bucket = 0;  // bucket sizes above this are vanishingly rare
     // device buffers
     KeyTile *tiles = nullptr;
     uint32_t *counts = nullptr, *size_hist = nullptr, *err = nullptr, *cursor = nullptr, *list_fill = nullptr,
@@ -269,7 +271,15 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
     scan_level(s, p->node_cnt, false, NB, p->node_base, p->scan_tmp);
     k_plan_census<<<1, 256, 0, s>>>(p->size_hist, p->census, p->list_count, p->list_base, p->err);
     k_scatter<<<p->n_tiles, 256, 0, s>>>(p->tiles, p->acc, p->cursor, p->sorted);
-    k_sort_buckets<<<NB, 256, (size_t)4096 * 8, s>>>(p->sorted, p->acc, p->err, rotate ? p->leaf : 0);
+    // buckets up to the likely largest size with a small LDS footprint, larger
+    // ones (possible, vanishingly rare) by a second pass with room for 4096 keys
+    uint32_t sort_p = 1;
+    while (sort_p < p->likely_max_bucket) sort_p <<= 1;
+    k_sort_buckets<<<NB, p->likely_max_bucket > 512 ? 256 : 64, (size_t)sort_p * 8, s>>>(
+        p->sorted, p->acc, p->err, rotate ? p->leaf : 0, 0, p->likely_max_bucket);
+    if (p->likely_max_bucket < kMaxBucket - 1)
+        k_sort_buckets<<<NB, 256, (size_t)4096 * 8, s>>>(p->sorted, p->acc, p->err, rotate ? p->leaf : 0,
+                                                         p->likely_max_bucket + 1, kMaxBucket - 1);
     ExpandArgs ea{P, rotate ? 1 : 0, p->pool, p->list_base, p->list_fill, p->node_m};
     k_expand<<<(NB + 63) / 64, 64, 0, s>>>(p->acc, p->node_base, NB, ea);
     HIP_CHECK(hipGetLastError());
@@ -278,15 +288,21 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
     const AggrLayout up_l = aggr_layout(P.lower, P.upper / P.lower);
     const AggrLayout lo_l = aggr_layout(P.leaf, P.lower / P.leaf);
     const Tune &tn = p->tune;
-    for (int l : p->lists) {
+    for (size_t li = 0; li < p->lists.size(); li++) {
+        const int l = p->lists[li];
+        // levels only buckets far above the likely largest size reach run on a
+        // small grid: correct if such a bucket appears, nearly free when none does
+        const bool likely = p->likely[li];
+        const uint32_t blocks = likely ? p->blocks : std::max<uint32_t>(1, p->blocks / 64);
         const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
-        k_level_init<<<fill_grid, 256, 0, s>>>(p->state, p->running, p->ctl, p->n_waves, p->list_count, (uint32_t)l);
+        k_level_init<<<likely ? fill_grid : 4, 256, 0, s>>>(p->state, p->running, p->ctl, blocks * 4, p->list_count,
+                                                            (uint32_t)l);
         SearchArgs a{};
         a.nodes = p->pool;
         a.state = p->state;
         a.running = p->running;
         a.ctl = p->ctl;
-        a.n_waves = p->n_waves;
+        a.n_waves = blocks * 4;
         a.leaf = P.leaf;
         a.upper = P.upper;
         a.work = p->work + l;
@@ -311,15 +327,15 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
-        launch_search(s, tn, kind, p->sorted, a, wide, lut, p->blocks, lds, r64);
+        launch_search(s, tn, kind, p->sorted, a, wide, lut, blocks, lds, r64);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? kMaxBucket : (kind == kUpper ? P.upper : P.lower);
-            k_split<<<(uint32_t)p->num_cus * 8, 64, (size_t)max_m * 8, s>>>(
+            k_split<<<likely ? (uint32_t)p->num_cus * 8 : 8, 64, (size_t)max_m * 8, s>>>(
                 p->sorted, p->pool, p->state, p->seeds, p->err, P, kind, r64 ? 1 : 0, p->list_count, p->list_base,
                 (uint32_t)l);
         } else {
-            k_finalize<<<fill_grid, 256, 0, s>>>(p->pool, p->state, 0, p->seeds, p->err, p->list_count, p->list_base,
-                                                 (uint32_t)l);
+            k_finalize<<<likely ? fill_grid : 4, 256, 0, s>>>(p->pool, p->state, 0, p->seeds, p->err, p->list_count,
+                                                              p->list_base, (uint32_t)l);
         }
         HIP_CHECK(hipGetLastError());
     }
@@ -432,10 +448,21 @@ int rs_plan_create(rs_ctx *c, size_t n, uint32_t leaf, size_t bucket, int flags,
         HIP_CHECK(hipMemcpy(p->memo + kMaxBucket, tree_nodes(leaf).data(), sizeof(uint32_t) * kMaxBucket,
                             hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(p->census, table.data(), table.size() * 4, hipMemcpyHostToDevice));
+        // bucket sizes are Poisson(n / nb) for uniform keys: 10 standard deviations
+        // above the mean (plus a margin) bounds every bucket of every practical build
+        const double mean = (double)n / (double)NB;
+        p->likely_max_bucket = (uint32_t)std::min<double>(kMaxBucket - 1, mean + 10.0 * std::sqrt(mean) + 32);
         for (int l = 0; l < kNumLists; l++) {
-            bool any = false;
-            for (uint32_t m = 2; m < kMaxBucket && !any; m++) any = table[(size_t)m * kNumLists + l] != 0;
-            if (any) p->lists.push_back(l);
+            bool any = false, lik = false;
+            for (uint32_t m = 2; m < kMaxBucket; m++)
+                if (table[(size_t)m * kNumLists + l]) {
+                    any = true;
+                    lik |= m <= p->likely_max_bucket;
+                }
+            if (any) {
+                p->lists.push_back(l);
+                p->likely.push_back(lik ? 1 : 0);
+            }
         }
     });
     if (rc != RS_OK) {
