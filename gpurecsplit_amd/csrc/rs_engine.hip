@@ -169,44 +169,85 @@ __global__ __launch_bounds__(256) void k_scan_add(uint64_t *__restrict__ out, ui
 // ------------------------------------------------------------------ census
 
 // Node counts per phase list for one bucket-size histogram (tree shapes follow
-// AbstractParallelRecSplit.hpp:137-148).
+// AbstractParallelRecSplit.hpp:137-148), with the expected key evaluations of
+// each list's search.
 struct Census {
     uint64_t count[kNumLists] = {};
+    double evals[kNumLists] = {};
     uint32_t max_depth = 0;
 };
 
-static void census_node(const SplitParams &P, bool rotate, uint32_t s, uint32_t depth, uint64_t mult, Census &c,
-                        std::vector<std::array<uint64_t, 4>> &small, std::vector<bool> &small_done) {
+// Subtree below an aggregation node or leaf of a given size: nodes and expected
+// evaluations per kind (the same at every depth).
+struct SubCensus {
+    bool done = false;
+    uint64_t n[4] = {};
+    double e[4] = {};
+};
+
+// Expected key evaluations of the search of one node of s keys.  The Golomb-Rice
+// parameter g of its size satisfies 2^g ~ -ln(phi) / ln(1 - p) for the success
+// probability p of one candidate (AbstractParallelRecSplit.hpp:191-227), so
+// 1/p ~ 2^g / 0.48; each candidate reads the s keys, and a rotation leaf tests
+// all l rotations of a candidate with one pass over its keys.
+static double node_evals(const Memo &memo, uint32_t s, uint32_t kind, uint32_t leaf) {
+    const double e = (double)s * std::ldexp(2.08, (int)golomb_param(memo, s));
+    return kind == kLeafRot ? e / leaf : e;
+}
+
+static void census_node(const SplitParams &P, const Memo &memo, bool rotate, uint32_t s, uint32_t depth,
+                        uint64_t mult, Census &c, std::vector<SubCensus> &small) {
     const uint32_t kind = classify(P, s, rotate);
     if (kind != kHigher) {
-        if (!small_done[s]) {
-            // subtree of an aggregation node or leaf: independent of depth
-            std::array<uint64_t, 4> acc{};
-            acc[kind - 1] += 1;
+        SubCensus &sub = small[s];
+        if (!sub.done) {
+            SubCensus acc;
+            acc.n[kind - 1] += 1;
+            acc.e[kind - 1] += node_evals(memo, s, kind, P.leaf);
             if (s > P.leaf) {
                 const uint32_t unit = kind == kUpper ? P.lower : P.leaf;
                 const uint32_t fan = (uint16_t)(s + unit - 1) / unit;
                 for (uint32_t i = 0; i < fan; i++) {
                     const uint32_t cs = (i + 1 == fan) ? s - (fan - 1) * unit : unit;
                     if (cs > 1) {
-                        Census sub;
-                        census_node(P, rotate, cs, 0, 1, sub, small, small_done);
-                        for (int k = 0; k < 4; k++) acc[k] += sub.count[kMaxDepth + k];
+                        Census part;
+                        census_node(P, memo, rotate, cs, 0, 1, part, small);
+                        for (int k = 0; k < 4; k++) {
+                            acc.n[k] += part.count[kMaxDepth + k];
+                            acc.e[k] += part.evals[kMaxDepth + k];
+                        }
                     }
                 }
             }
-            small[s] = acc;
-            small_done[s] = true;
+            acc.done = true;
+            sub = acc;
         }
-        for (int k = 0; k < 4; k++) c.count[kMaxDepth + k] += small[s][k] * mult;
+        for (int k = 0; k < 4; k++) {
+            c.count[kMaxDepth + k] += sub.n[k] * mult;
+            c.evals[kMaxDepth + k] += sub.e[k] * (double)mult;
+        }
         return;
     }
     if (depth >= kMaxDepth) throw BuildError(RS_ERR_ARG, "split tree deeper than the start-seed table");
     c.count[depth] += mult;
+    c.evals[depth] += node_evals(memo, s, kHigher, P.leaf) * (double)mult;
     c.max_depth = std::max(c.max_depth, depth);
     const uint32_t split = higher_split(P, s);
-    census_node(P, rotate, split, depth + 1, mult, c, small, small_done);
-    if (s - split > 1) census_node(P, rotate, s - split, depth + 1, mult, c, small, small_done);
+    census_node(P, memo, rotate, split, depth + 1, mult, c, small);
+    if (s - split > 1) census_node(P, memo, rotate, s - split, depth + 1, mult, c, small);
+}
+
+// Workgroups of one search launch.  A wave costs more the more waves share the
+// level's nodes (claims and result checks on the nodes' counters, scans of the
+// running table by waves without a node), so a level of little work runs on fewer
+// waves.  Fitted on MI355X (l = 8..16, 10K..1M keys): w = sqrt((E + 5000 n) /
+// 1000) waves for E expected evaluations over n nodes (5000 evaluations stand for
+// a node's fixed cost), at least n / 8; levels of more than ~6e10 evaluations or
+// 64K nodes take the whole grid.
+static uint32_t level_blocks(double evals, double nodes, uint32_t full_blocks) {
+    const double waves = std::max(std::sqrt((std::max(evals, 0.0) + 5000.0 * nodes) / 1000.0), nodes / 8.0);
+    const double b = std::ceil(waves / 4.0);
+    return (uint32_t)std::max(std::min(b, (double)full_blocks), std::min(8.0, (double)full_blocks));
 }
 
 }  // namespace rsa
@@ -587,12 +628,11 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     if (h[kMaxBucket]) throw BuildError(RS_ERR_BUCKET_TOO_LARGE, "a bucket holds 3000 or more keys");
     Census cen;
     {
-        std::vector<std::array<uint64_t, 4>> small(kMaxBucket);
-        std::vector<bool> small_done(kMaxBucket, false);
+        std::vector<SubCensus> small(kMaxBucket);
         for (uint32_t m = 2; m < kMaxBucket; m++) {
             if (!h[m]) continue;
             st.max_bucket_size = m;
-            census_node(P, rotate, m, 0, h[m], cen, small, small_done);
+            census_node(P, hmemo, rotate, m, 0, h[m], cen, small);
         }
         if (!st.max_bucket_size) st.max_bucket_size = h[1] ? 1 : 0;
     }
@@ -667,8 +707,10 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         if (!cnt) continue;
         const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
         const NodeDesc *nodes = pool + lbase[l];
+        const uint32_t lblocks = level_blocks(cen.evals[l], (double)cnt, blocks);
+        const uint32_t lwaves = lblocks * 4;
         k_state_init<<<(cnt + 255) / 256, 256, 0, s>>>(state, cnt);
-        HIP_CHECK(hipMemsetAsync(running, 0xFF, n_waves * 4, s));
+        HIP_CHECK(hipMemsetAsync(running, 0xFF, lwaves * 4, s));
         HIP_CHECK(hipMemsetAsync(ctl, 0, 64, s));
         SearchArgs a{};
         a.nodes = nodes;
@@ -676,7 +718,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.running = running;
         a.ctl = ctl;
         a.n_nodes = cnt;
-        a.n_waves = n_waves;
+        a.n_waves = lwaves;
         a.leaf = P.leaf;
         a.upper = P.upper;
         a.work = work + l;
@@ -688,7 +730,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.help_cap = (uint32_t)tn.help_cap[kind];
         // Levels of many small nodes (millions of leaves of many small sets) would
         // otherwise be bound by the one fresh-node counter (one atomic per node).
-        claim_params(cnt, n_waves, a.claim_batch, a.batch_tickets);
+        claim_params(cnt, lwaves, a.claim_batch, a.batch_tickets);
         bool wide = false, lut = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
@@ -706,7 +748,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             if (!lut) lds = 0;
         }
         PhaseEvents pe{kind, ev_record(c), -1, -1};
-        launch_search(s, tn, kind, sorted, a, wide, lut, blocks, lds, r64);
+        launch_search(s, tn, kind, sorted, a, wide, lut, lblocks, lds, r64);
         pe.search1 = ev_record(c);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);

@@ -976,7 +976,7 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
     }
     if (lane == 0) {
         __hip_atomic_store(&a.running[wave], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        atomicAdd(a.work, (unsigned long long)work);
+        if (work) atomicAdd(a.work, (unsigned long long)work);
         if (a.wave_clock) a.wave_clock[2 * wave + 1] = (unsigned long long)wall_clock64();
     }
 }

@@ -211,6 +211,7 @@ struct rs_plan {
     uint32_t blocks = 0, n_waves = 0, n_tiles = 0;
     std::vector<int> lists;     // tree levels some bucket size can have
     std::vector<char> likely;   // ... and whether a bucket of the likely sizes has it
+    std::vector<uint32_t> level_blocks;  // ... and its search grid (expected work)
     uint32_t likely_max_bucket = 0;  // bucket sizes above this are vanishingly rare
     // device buffers
     KeyTile *tiles = nullptr;
@@ -236,15 +237,20 @@ struct rs_plan {
 namespace rsa {
 
 // Node counts per list of one bucket of each size m < kMaxBucket (the census of
-// the ordinary build, one bucket at a time).
-static std::vector<uint32_t> census_table(const SplitParams &P, bool rotate) {
+// the ordinary build, one bucket at a time), and the expected key evaluations of
+// those nodes' searches.
+static std::vector<uint32_t> census_table(const SplitParams &P, bool rotate, std::vector<double> &evals) {
     std::vector<uint32_t> t((size_t)kMaxBucket * kNumLists, 0);
-    std::vector<std::array<uint64_t, 4>> small(kMaxBucket);
-    std::vector<bool> small_done(kMaxBucket, false);
+    evals.assign((size_t)kMaxBucket * kNumLists, 0.0);
+    const Memo &memo = golomb_memo(P.leaf);
+    std::vector<SubCensus> small(kMaxBucket);
     for (uint32_t m = 2; m < kMaxBucket; m++) {
         Census c;
-        census_node(P, rotate, m, 0, 1, c, small, small_done);
-        for (int l = 0; l < kNumLists; l++) t[(size_t)m * kNumLists + l] = (uint32_t)c.count[l];
+        census_node(P, memo, rotate, m, 0, 1, c, small);
+        for (int l = 0; l < kNumLists; l++) {
+            t[(size_t)m * kNumLists + l] = (uint32_t)c.count[l];
+            evals[(size_t)m * kNumLists + l] = c.evals[l];
+        }
     }
     return t;
 }
@@ -293,7 +299,7 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
         // levels only buckets far above the likely largest size reach run on a
         // small grid: correct if such a bucket appears, nearly free when none does
         const bool likely = p->likely[li];
-        const uint32_t blocks = likely ? p->blocks : std::max<uint32_t>(1, p->blocks / 64);
+        const uint32_t blocks = likely ? p->level_blocks[li] : std::max<uint32_t>(1, p->blocks / 64);
         const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
         k_level_init<<<likely ? fill_grid : 4, 256, 0, s>>>(p->state, p->running, p->ctl, blocks * 4, p->list_count,
                                                             (uint32_t)l);
@@ -443,7 +449,8 @@ int rs_plan_create(rs_ctx *c, size_t n, uint32_t leaf, size_t bucket, int flags,
         real.size = p->mem_bytes;
         plan_layout(real);
         // tables, and the tree levels some bucket size can reach
-        const std::vector<uint32_t> table = census_table(split_params(leaf), (flags & kFlagRotate) != 0);
+        std::vector<double> evals;
+        const std::vector<uint32_t> table = census_table(split_params(leaf), (flags & kFlagRotate) != 0, evals);
         HIP_CHECK(hipMemcpy(p->memo, golomb_memo(leaf).data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(p->memo + kMaxBucket, tree_nodes(leaf).data(), sizeof(uint32_t) * kMaxBucket,
                             hipMemcpyHostToDevice));
@@ -452,16 +459,25 @@ int rs_plan_create(rs_ctx *c, size_t n, uint32_t leaf, size_t bucket, int flags,
         // above the mean (plus a margin) bounds every bucket of every practical build
         const double mean = (double)n / (double)NB;
         p->likely_max_bucket = (uint32_t)std::min<double>(kMaxBucket - 1, mean + 10.0 * std::sqrt(mean) + 32);
+        // expected evaluations per list over Poisson bucket sizes size each
+        // level's grid as the ordinary build does from its census
+        std::vector<double> pmf(kMaxBucket, 0.0);
+        for (uint32_t m = 0; m < kMaxBucket; m++)
+            pmf[m] = std::exp((double)m * std::log(mean) - mean - std::lgamma((double)m + 1.0));
         for (int l = 0; l < kNumLists; l++) {
             bool any = false, lik = false;
+            double e = 0, cnt = 0;
             for (uint32_t m = 2; m < kMaxBucket; m++)
                 if (table[(size_t)m * kNumLists + l]) {
                     any = true;
                     lik |= m <= p->likely_max_bucket;
+                    e += pmf[m] * evals[(size_t)m * kNumLists + l];
+                    cnt += pmf[m] * table[(size_t)m * kNumLists + l];
                 }
             if (any) {
                 p->lists.push_back(l);
                 p->likely.push_back(lik ? 1 : 0);
+                p->level_blocks.push_back(rsa::level_blocks(e * (double)NB, cnt * (double)NB, p->blocks));
             }
         }
     });
