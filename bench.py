@@ -250,10 +250,12 @@ def plan_graph(gra, ctx, splitmix64_keys_device, reps: int = 20) -> dict:
     bytes in host memory at the end: the ordinary build (the host reads sizes back
     between phases), a build plan (rs_plan_enqueue: one queued build, one
     synchronisation) and a replay of the plan captured in a graph.  All three
-    give the same bytes."""
+    give the same bytes.  Each entry also gives the ordinary build's search ms and
+    launch waves per kind (levels of little work run on fewer waves)."""
     import torch
     out = {}
-    for n, leaf, b in [(10_000, 8, 100), (100_000, 12, 100), (100_000, 16, 2000), (1_000_000, 8, 100)]:
+    for n, leaf, b in [(10_000, 8, 100), (10_000, 12, 100), (100_000, 12, 100), (10_000, 16, 2000),
+                       (100_000, 16, 2000), (1_000_000, 8, 100)]:
         keys = splitmix64_keys_device(20 + leaf, n)
         torch.cuda.synchronize()
 
@@ -274,6 +276,7 @@ def plan_graph(gra, ctx, splitmix64_keys_device, reps: int = 20) -> dict:
 
         eager()
         e_ms, ref = timed(eager)
+        st = gra.GPURecSplit(keys, b, leaf_size=leaf, ctx=ctx).stats()
         plan = gra.Plan(n, b, leaf, ctx=ctx)
         plan.build(keys)
         p_ms, p_data = timed(lambda: plan.build(keys))
@@ -292,7 +295,9 @@ def plan_graph(gra, ctx, splitmix64_keys_device, reps: int = 20) -> dict:
         g_ms, g_data = timed(replay)
         out[f"n{n}_l{leaf}_b{b}"] = {"build_ms": e_ms, "plan_ms": p_ms, "graph_replay_ms": g_ms,
                                      "same_bytes": p_data == ref and g_data == ref,
-                                     "plan_device_bytes": plan.device_bytes}
+                                     "plan_device_bytes": plan.device_bytes,
+                                     "search_ms": {k: v for k, v in st["search_ms"].items() if v > 0},
+                                     "kind_waves": {k: v for k, v in st["kind_waves"].items() if v > 0}}
         log(f"plan/graph n={n} l={leaf} b={b}: build {e_ms:.2f} ms, plan {p_ms:.2f} ms, graph {g_ms:.2f} ms")
         del g
         plan.close()

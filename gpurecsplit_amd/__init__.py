@@ -58,6 +58,7 @@ class BuildStats(C.Structure):
         ("nodes", C.c_uint64 * 8), ("executed_evals", C.c_uint64 * 8), ("num_buckets", C.c_uint64), ("num_nodes", C.c_uint64),
         ("rice_bits", C.c_uint64), ("max_bucket_size", C.c_uint32), ("search_waves", C.c_uint32),
         ("search_busy", C.c_double * 8), ("search_tail_ms", C.c_double * 8), ("seeds", C.c_uint64 * 8),
+        ("kind_waves", C.c_uint32 * 8),
     ]
 
     def as_dict(self) -> dict:
@@ -74,6 +75,7 @@ class BuildStats(C.Structure):
             "search_busy": {k: self.search_busy[i] for i, k in enumerate(KIND_NAMES)},
             "search_tail_ms": {k: self.search_tail_ms[i] for i, k in enumerate(KIND_NAMES)},
             "seeds": {k: int(self.seeds[i]) for i, k in enumerate(KIND_NAMES)},
+            "kind_waves": {k: int(self.kind_waves[i]) for i, k in enumerate(KIND_NAMES)},
         }
 
 

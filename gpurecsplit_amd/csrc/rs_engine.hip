@@ -709,6 +709,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         const NodeDesc *nodes = pool + lbase[l];
         const uint32_t lblocks = level_blocks(cen.evals[l], (double)cnt, blocks);
         const uint32_t lwaves = lblocks * 4;
+        st.kind_waves[kind] = std::max(st.kind_waves[kind], lwaves);
         k_state_init<<<(cnt + 255) / 256, 256, 0, s>>>(state, cnt);
         HIP_CHECK(hipMemsetAsync(running, 0xFF, lwaves * 4, s));
         HIP_CHECK(hipMemsetAsync(ctl, 0, 64, s));

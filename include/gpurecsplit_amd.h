@@ -85,7 +85,7 @@ typedef struct rs_build_stats {
     uint64_t num_nodes;
     uint64_t rice_bits;
     uint32_t max_bucket_size;
-    uint32_t search_waves;    /* persistent waves per search launch */
+    uint32_t search_waves;    /* persistent waves of the full search grid */
     /* Per kind (same index), filled when the context was created with the
      * environment variable RS_WAVE_TIMES=1 (else 0): every search wave records its
      * start and exit on the device clock.  search_busy = share of the resident
@@ -98,6 +98,9 @@ typedef struct rs_build_stats {
      * per split or brute-force node with seed x, (floor(x/L) + 1) * L per rotation
      * leaf (each pass over its L keys tests L rotations, i.e. L seeds) */
     uint64_t seeds[8];
+    /* per kind: the most persistent waves a launch of that kind ran with (levels
+     * of little expected work run on fewer than search_waves) */
+    uint32_t kind_waves[8];
 } rs_build_stats;
 
 /* Context bound to one GPU (device ordinal as seen by HIP). */

@@ -324,3 +324,17 @@ def test_seed_counts_from_the_seeds(torch_cuda):
     assert i == len(seeds)
     assert sum(st["seeds"].values()) == expect
     assert st["seeds"]["leaf_rotation"] == st["evals"]["leaf_rotation"]
+
+
+def test_small_levels_run_on_fewer_waves(torch_cuda):
+    """Levels of little expected work launch fewer waves than the full grid, and
+    the bytes do not depend on the grid (they equal the oracle's)."""
+    keys = splitmix_keys(4040, 10_000)
+    _log("small levels leaf=8 bucket=100 n=10000")
+    rs = gra.GPURecSplit(keys, 100, leaf_size=8)
+    st = rs.stats()
+    full = st["search_waves"]
+    waves = {k: v for k, v in st["kind_waves"].items() if st["nodes"][k]}
+    assert waves and all(0 < v <= full for v in waves.values()), (waves, full)
+    assert min(waves.values()) < full
+    assert rs.serialize() == OracleMPHF.build(keys, 8, 100, threads=8).serialize()
