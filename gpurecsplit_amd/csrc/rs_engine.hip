@@ -174,6 +174,7 @@ __global__ __launch_bounds__(256) void k_scan_add(uint64_t *__restrict__ out, ui
 struct Census {
     uint64_t count[kNumLists] = {};
     double evals[kNumLists] = {};
+    uint64_t full[kNumLists] = {};  // aggregation nodes of the level's full size
     uint32_t max_depth = 0;
 };
 
@@ -183,6 +184,7 @@ struct SubCensus {
     bool done = false;
     uint64_t n[4] = {};
     double e[4] = {};
+    uint64_t f[4] = {};
 };
 
 // Expected key evaluations of the search of one node of s keys.  The Golomb-Rice
@@ -204,6 +206,7 @@ static void census_node(const SplitParams &P, const Memo &memo, bool rotate, uin
             SubCensus acc;
             acc.n[kind - 1] += 1;
             acc.e[kind - 1] += node_evals(memo, s, kind, P.leaf);
+            acc.f[kind - 1] += (kind == kUpper && s == P.upper) || (kind == kLower && s == P.lower);
             if (s > P.leaf) {
                 const uint32_t unit = kind == kUpper ? P.lower : P.leaf;
                 const uint32_t fan = (uint16_t)(s + unit - 1) / unit;
@@ -215,6 +218,7 @@ static void census_node(const SplitParams &P, const Memo &memo, bool rotate, uin
                         for (int k = 0; k < 4; k++) {
                             acc.n[k] += part.count[kMaxDepth + k];
                             acc.e[k] += part.evals[kMaxDepth + k];
+                            acc.f[k] += part.full[kMaxDepth + k];
                         }
                     }
                 }
@@ -225,6 +229,7 @@ static void census_node(const SplitParams &P, const Memo &memo, bool rotate, uin
         for (int k = 0; k < 4; k++) {
             c.count[kMaxDepth + k] += sub.n[k] * mult;
             c.evals[kMaxDepth + k] += sub.e[k] * (double)mult;
+            c.full[kMaxDepth + k] += sub.f[k] * mult;
         }
         return;
     }
@@ -260,7 +265,9 @@ using namespace rsa;
 // lane for every kind, uK: for kind K only, lut: LDS increment table for aggregation,
 // sk / skK: keys in scalar registers for every kind / kind K,
 // gK: 64*u-candidate groups per claimed chunk of kind K, hK: helper cap of kind K
-// (0: none), bpc: workgroups of 4 waves per CU).  Fanout-2 nodes need ~40 candidates, so they run one per lane.
+// (0: none), bpc: workgroups of 4 waves per CU, bt: byte table for aggregation
+// levels of mostly full-size nodes, ubt: candidates per lane there).  Fanout-2
+// nodes need ~40 candidates, so they run one per lane.
 struct Tune {
     int u[kNumKinds] = {1, 2, 2, 4, 4};
     int sk[kNumKinds] = {1, 1, 1, 1, 1};  // keys in scalar registers
@@ -270,6 +277,8 @@ struct Tune {
     // fanout-2 nodes need ~40 candidates, so extra waves mostly overshoot
     int help_cap[kNumKinds] = {2, 0, 0, 0, 0};
     int blocks_per_cu = 8;
+    int bt = 1;      // byte table for the full-size nodes of aggregation levels
+    int u_bt = 4;    // candidates in flight per lane in byte-table launches
 };
 
 static Tune parse_tune(const char *env, Tune t = Tune()) {
@@ -290,6 +299,8 @@ static Tune parse_tune(const char *env, Tune t = Tune()) {
             else if (k.size() == 2 && k[0] == 'u' && k[1] >= '0' && k[1] < '0' + (int)kNumKinds && uv)
                 t.u[k[1] - '0'] = v;
             else if (k == "lut") t.lut = v != 0;
+            else if (k == "bt") t.bt = v != 0;
+            else if (k == "ubt" && uv) t.u_bt = v;
             else if (k == "sk")
                 for (int &x : t.sk) x = v != 0;
             else if (k.size() == 3 && k[0] == 's' && k[1] == 'k' && k[2] >= '0' && k[2] < '0' + (int)kNumKinds)
@@ -314,6 +325,8 @@ static std::string format_tune(const Tune &t) {
     for (int k = 0; k < kNumKinds; k++) add("sk" + std::to_string(k), t.sk[k]);
     for (int k = 0; k < kNumKinds; k++) add("h" + std::to_string(k), t.help_cap[k]);
     add("lut", t.lut);
+    add("bt", t.bt);
+    add("ubt", t.u_bt);
     add("bpc", t.blocks_per_cu);
     return s;
 }
@@ -443,8 +456,10 @@ static double ev_ms(rs_ctx *c, int a, int b) {
 template <int KIND, int U, bool SK, bool R64>
 static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
                         size_t lds) {
-    if (KIND == kUpper || KIND == kLower) {
-        if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
+    if constexpr (KIND == kUpper || KIND == kLower) {
+        if (a.bt_m && !wide && lut && !R64)
+            k_search<KIND, U, false, true, SK, false, false, true><<<blocks, 512, lds, s>>>(keys, a);
+        else if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
@@ -477,10 +492,26 @@ static void launch_sk(hipStream_t s, int u, uint32_t kind, const uint64_t *keys,
     }
 }
 
+// Grid of one search level from its size in 4-wave workgroups: the byte-table
+// instance (aggregation levels with 32-bit counters and the 16-bit remap whose
+// nodes are mostly of the full size) runs 8-wave workgroups, two per CU, since its
+// 64 KiB table is per workgroup.
+struct SearchShape {
+    uint32_t grid, waves;
+    bool bt;
+};
+static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool lut, bool r64, bool mostly_full,
+                                uint32_t blocks4, uint32_t full_blocks4) {
+    if (!(tn.bt && (kind == kUpper || kind == kLower) && lut && !wide && !r64 && mostly_full))
+        return {blocks4, blocks4 * 4, false};
+    const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 + 1) / 2, full_blocks4 / 2));  // never past the full grid
+    return {g, g * 8, true};
+}
+
 // The 64-bit remap runs with keys in scalar registers only.
 static void launch_search(hipStream_t s, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
                           bool wide, bool lut, uint32_t blocks, size_t lds, bool r64) {
-    const int u = tn.u[kind];
+    const int u = a.bt_m ? tn.u_bt : tn.u[kind];
     if (r64) launch_sk<true, true>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else if (tn.sk[kind]) launch_sk<true, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else launch_sk<false, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
@@ -708,10 +739,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
         const NodeDesc *nodes = pool + lbase[l];
         const uint32_t lblocks = level_blocks(cen.evals[l], (double)cnt, blocks);
-        const uint32_t lwaves = lblocks * 4;
-        st.kind_waves[kind] = std::max(st.kind_waves[kind], lwaves);
         k_state_init<<<(cnt + 255) / 256, 256, 0, s>>>(state, cnt);
-        HIP_CHECK(hipMemsetAsync(running, 0xFF, lwaves * 4, s));
         HIP_CHECK(hipMemsetAsync(ctl, 0, 64, s));
         SearchArgs a{};
         a.nodes = nodes;
@@ -719,7 +747,6 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.running = running;
         a.ctl = ctl;
         a.n_nodes = cnt;
-        a.n_waves = lwaves;
         a.leaf = P.leaf;
         a.upper = P.upper;
         a.work = work + l;
@@ -729,9 +756,6 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
         a.wave_clock = wave_clock ? wave_clock + (size_t)l * 2 * n_waves : nullptr;
         a.groups = (uint32_t)tn.groups[kind];
         a.help_cap = (uint32_t)tn.help_cap[kind];
-        // Levels of many small nodes (millions of leaves of many small sets) would
-        // otherwise be bound by the one fresh-node counter (one atomic per node).
-        claim_params(cnt, lwaves, a.claim_batch, a.batch_tickets);
         bool wide = false, lut = false;
         size_t lds = 0;
         if (kind == kUpper || kind == kLower) {
@@ -748,8 +772,17 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
+        const SearchShape sh = search_shape(tn, kind, wide, lut, r64, 2 * cen.full[l] >= cnt, lblocks, blocks);
+        a.bt_m = sh.bt ? a.lut_entries : 0;
+        a.n_waves = sh.waves;
+        if (sh.waves > n_waves) throw BuildError(RS_ERR_INTERNAL, "search launch beyond the running table");
+        HIP_CHECK(hipMemsetAsync(running, 0xFF, (size_t)sh.waves * 4, s));
+        st.kind_waves[kind] = std::max(st.kind_waves[kind], sh.waves);
+        // Levels of many small nodes (millions of leaves of many small sets) would
+        // otherwise be bound by the one fresh-node counter (one atomic per node).
+        claim_params(cnt, sh.waves, a.claim_batch, a.batch_tickets);
         PhaseEvents pe{kind, ev_record(c), -1, -1};
-        launch_search(s, tn, kind, sorted, a, wide, lut, lblocks, lds, r64);
+        launch_search(s, tn, kind, sorted, a, wide, lut, sh.grid, lds, r64);
         pe.search1 = ev_record(c);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? st.max_bucket_size : (kind == kUpper ? P.upper : P.lower);
@@ -1102,6 +1135,7 @@ static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags
                 t.u[k] = u;
                 t.groups[k] = g;
             }
+            t.u_bt = u;
             const rs_build_stats st = timed(t);
             rep += std::string(first ? "" : ",") + "{\"u\":" + std::to_string(u) + ",\"g\":" + std::to_string(g) +
                    ",\"search_ms\":[";
@@ -1111,6 +1145,7 @@ static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags
                     best_ms[k] = st.search_ms[k];
                     best.u[k] = u;
                     best.groups[k] = g;
+                    if (k == kLower) best.u_bt = u;  // byte-table launches (lower level dominates)
                 }
             }
             rep += "]}";

@@ -364,6 +364,10 @@ struct SearchArgs {
     // (duplicate keys or an oversized bucket found by the partition).
     const uint32_t *stop;
     uint32_t stop_mask;
+    // Aggregation levels: nodes of exactly bt_m keys (the level's full size, most
+    // of its nodes) read their counter shift from a byte table indexed by the
+    // 16-bit hash (0: no such table; the kernel instance BT builds it).
+    uint32_t bt_m;
 };
 
 // Node claim parameters of a level of n_nodes nodes for n_waves waves: tickets take
@@ -595,6 +599,37 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
     return kNone;
 }
 
+// Byte table of the BT instance: entry h (a 16-bit hash) holds fb * part for the
+// part floor(((h * bt_m) >> 16) / SPLIT) of a node of bt_m keys, so one LDS byte
+// read replaces the remap and the increment table lookup.  65536 entries in the
+// workgroup's static LDS (at most two 512-thread workgroups per CU).
+template <bool BT>
+__device__ __forceinline__ const uint8_t *bt_table() {
+    if constexpr (BT) {
+        __shared__ uint32_t s_bt[65536 / 4];
+        return (const uint8_t *)s_bt;
+    } else {
+        return nullptr;
+    }
+}
+
+// Aggregation node of the byte table's size: the same predicate as group_aggr,
+// the increment of every key 1 << bt[h16] (h16 < 2^16, 32-bit counters).
+template <int U, bool SK>
+__device__ __forceinline__ uint64_t group_aggr_bt(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
+                                                  const uint8_t *bt, uint64_t T, uint64_t cmask, uint64_t u0) {
+    uint32_t cnt[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) cnt[u] = 0;
+    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += 1u << bt[h]; });
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const unsigned long long msk = __ballot(((uint64_t)cnt[u] & cmask) == T);
+        if (msk) return u0 + (uint64_t)(u * 64 + __ffsll((long long)msk) - 1);
+    }
+    return kNone;
+}
+
 // Full leaf with rotation fitting (SIMDRecSplit.hpp:452-509): unit j is the base
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
@@ -735,9 +770,14 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
 }
 
 // RB (rotation leaves of at most 16 keys): packed masks and the candidate test
-// batched through LDS, with registers held to five waves per SIMD.
-template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false>
-__global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__restrict__ keys, SearchArgs a) {
+// batched through LDS, with registers held to five waves per SIMD.  BT
+// (aggregation levels, 32-bit counters, 16-bit remap): the byte table for nodes of
+// bt_m keys, workgroups of 512 threads, two per CU.
+template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
+__global__ __launch_bounds__(BT ? 512 : 256, RB ? 5 : (BT ? 2 : 1)) void k_search(const uint64_t *__restrict__ keys,
+                                                                                  SearchArgs a) {
+    static_assert(!BT || ((KIND == kUpper || KIND == kLower) && LUT && !WIDE && !R64 && !RB),
+                  "the byte table serves 32-bit aggregation counters over the 16-bit remap");
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -761,6 +801,19 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
         for (uint32_t r = threadIdx.x; r < a.lut_entries; r += blockDim.x) {
             if (WIDE) ((uint64_t *)s_lut)[r] = 1ull << (a.field_bits * (r / a.split_unit));
             else s_lut[r] = 1u << (a.field_bits * (r / a.split_unit));
+        }
+        if constexpr (BT) {
+            uint32_t *bt_w = (uint32_t *)bt_table<BT>();
+            const uint32_t bm = a.bt_m, fbits = a.field_bits, magic = a.magic;
+            for (uint32_t w = threadIdx.x; w < 65536 / 4; w += blockDim.x) {
+                uint32_t v = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    const uint32_t r = ((w * 4 + j) * bm) >> 16;  // < bm < 3000
+                    v |= (fbits * ((r * magic) >> 20)) << (8 * j);  // r / SPLIT
+                }
+                bt_w[w] = v;
+            }
         }
         __syncthreads();
     }
@@ -956,6 +1009,8 @@ __global__ __launch_bounds__(256, RB ? 5 : 1) void k_search(const uint64_t *__re
             }
             work += (uint64_t)U * 64 * m;
             if (KIND == kHigher) hit = group_higher<U, SK, R64>(kp, m, sconst, aux0, aux1, u0);
+            else if ((KIND == kUpper || KIND == kLower) && BT && m == a.bt_m)
+                hit = group_aggr_bt<U, SK>(kp, m, sconst, bt_table<BT>(), T, cmask, u0);
             else if (KIND == kUpper || KIND == kLower)
                 hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
                                                         u0);

@@ -212,6 +212,7 @@ struct rs_plan {
     std::vector<int> lists;     // tree levels some bucket size can have
     std::vector<char> likely;   // ... and whether a bucket of the likely sizes has it
     std::vector<uint32_t> level_blocks;  // ... and its search grid (expected work)
+    std::vector<char> mostly_full;       // ... and whether most of its nodes are of full size
     uint32_t likely_max_bucket = 0;  // bucket sizes above this are vanishingly rare
     // device buffers
     KeyTile *tiles = nullptr;
@@ -237,11 +238,13 @@ struct rs_plan {
 namespace rsa {
 
 // Node counts per list of one bucket of each size m < kMaxBucket (the census of
-// the ordinary build, one bucket at a time), and the expected key evaluations of
-// those nodes' searches.
-static std::vector<uint32_t> census_table(const SplitParams &P, bool rotate, std::vector<double> &evals) {
+// the ordinary build, one bucket at a time), the expected key evaluations of
+// those nodes' searches, and how many of them are aggregation nodes of full size.
+static std::vector<uint32_t> census_table(const SplitParams &P, bool rotate, std::vector<double> &evals,
+                                          std::vector<uint32_t> &full) {
     std::vector<uint32_t> t((size_t)kMaxBucket * kNumLists, 0);
     evals.assign((size_t)kMaxBucket * kNumLists, 0.0);
+    full.assign((size_t)kMaxBucket * kNumLists, 0);
     const Memo &memo = golomb_memo(P.leaf);
     std::vector<SubCensus> small(kMaxBucket);
     for (uint32_t m = 2; m < kMaxBucket; m++) {
@@ -250,6 +253,7 @@ static std::vector<uint32_t> census_table(const SplitParams &P, bool rotate, std
         for (int l = 0; l < kNumLists; l++) {
             t[(size_t)m * kNumLists + l] = (uint32_t)c.count[l];
             evals[(size_t)m * kNumLists + l] = c.evals[l];
+            full[(size_t)m * kNumLists + l] = (uint32_t)c.full[l];
         }
     }
     return t;
@@ -301,14 +305,11 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
         const bool likely = p->likely[li];
         const uint32_t blocks = likely ? p->level_blocks[li] : std::max<uint32_t>(1, p->blocks / 64);
         const uint32_t kind = l < (int)kMaxDepth ? kHigher : (uint32_t)(l - kMaxDepth + 1);
-        k_level_init<<<likely ? fill_grid : 4, 256, 0, s>>>(p->state, p->running, p->ctl, blocks * 4, p->list_count,
-                                                            (uint32_t)l);
         SearchArgs a{};
         a.nodes = p->pool;
         a.state = p->state;
         a.running = p->running;
         a.ctl = p->ctl;
-        a.n_waves = blocks * 4;
         a.leaf = P.leaf;
         a.upper = P.upper;
         a.work = p->work + l;
@@ -333,7 +334,12 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
-        launch_search(s, tn, kind, p->sorted, a, wide, lut, blocks, lds, r64);
+        const SearchShape sh = search_shape(tn, kind, wide, lut, r64, p->mostly_full[li] != 0, blocks, p->blocks);
+        a.bt_m = sh.bt ? a.lut_entries : 0;
+        a.n_waves = sh.waves;
+        k_level_init<<<likely ? fill_grid : 4, 256, 0, s>>>(p->state, p->running, p->ctl, sh.waves, p->list_count,
+                                                            (uint32_t)l);
+        launch_search(s, tn, kind, p->sorted, a, wide, lut, sh.grid, lds, r64);
         if (kind == kHigher || kind == kUpper || kind == kLower) {
             const uint32_t max_m = kind == kHigher ? kMaxBucket : (kind == kUpper ? P.upper : P.lower);
             k_split<<<likely ? (uint32_t)p->num_cus * 8 : 8, 64, (size_t)max_m * 8, s>>>(
@@ -450,7 +456,8 @@ int rs_plan_create(rs_ctx *c, size_t n, uint32_t leaf, size_t bucket, int flags,
         plan_layout(real);
         // tables, and the tree levels some bucket size can reach
         std::vector<double> evals;
-        const std::vector<uint32_t> table = census_table(split_params(leaf), (flags & kFlagRotate) != 0, evals);
+        std::vector<uint32_t> full;
+        const std::vector<uint32_t> table = census_table(split_params(leaf), (flags & kFlagRotate) != 0, evals, full);
         HIP_CHECK(hipMemcpy(p->memo, golomb_memo(leaf).data(), sizeof(uint32_t) * kMaxBucket, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(p->memo + kMaxBucket, tree_nodes(leaf).data(), sizeof(uint32_t) * kMaxBucket,
                             hipMemcpyHostToDevice));
@@ -466,18 +473,20 @@ int rs_plan_create(rs_ctx *c, size_t n, uint32_t leaf, size_t bucket, int flags,
             pmf[m] = std::exp((double)m * std::log(mean) - mean - std::lgamma((double)m + 1.0));
         for (int l = 0; l < kNumLists; l++) {
             bool any = false, lik = false;
-            double e = 0, cnt = 0;
+            double e = 0, cnt = 0, nfull = 0;
             for (uint32_t m = 2; m < kMaxBucket; m++)
                 if (table[(size_t)m * kNumLists + l]) {
                     any = true;
                     lik |= m <= p->likely_max_bucket;
                     e += pmf[m] * evals[(size_t)m * kNumLists + l];
                     cnt += pmf[m] * table[(size_t)m * kNumLists + l];
+                    nfull += pmf[m] * full[(size_t)m * kNumLists + l];
                 }
             if (any) {
                 p->lists.push_back(l);
                 p->likely.push_back(lik ? 1 : 0);
                 p->level_blocks.push_back(rsa::level_blocks(e * (double)NB, cnt * (double)NB, p->blocks));
+                p->mostly_full.push_back(2 * nfull >= cnt ? 1 : 0);
             }
         }
     });
