@@ -458,7 +458,7 @@ static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a
                         size_t lds) {
     if constexpr (KIND == kUpper || KIND == kLower) {
         if (a.bt_m && !wide && lut && !R64)
-            k_search<KIND, U, false, true, SK, false, false, true><<<blocks, 512, lds, s>>>(keys, a);
+            k_search<KIND, U, false, true, SK, false, false, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
         else if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
@@ -494,8 +494,8 @@ static void launch_sk(hipStream_t s, int u, uint32_t kind, const uint64_t *keys,
 
 // Grid of one search level from its size in 4-wave workgroups: the byte-table
 // instance (aggregation levels with 32-bit counters and the 16-bit remap whose
-// nodes are mostly of the full size) runs 8-wave workgroups, two per CU, since its
-// 64 KiB table is per workgroup.
+// nodes are mostly of the full size) runs kBtThreads-thread workgroups, two per CU,
+// since its 64 KiB table is per workgroup.
 struct SearchShape {
     uint32_t grid, waves;
     bool bt;
@@ -504,8 +504,9 @@ static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool l
                                 uint32_t blocks4, uint32_t full_blocks4) {
     if (!(tn.bt && (kind == kUpper || kind == kLower) && lut && !wide && !r64 && mostly_full))
         return {blocks4, blocks4 * 4, false};
-    const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 + 1) / 2, full_blocks4 / 2));  // never past the full grid
-    return {g, g * 8, true};
+    const uint32_t wpg = kBtThreads / 64;  // waves per workgroup
+    const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 * 4 + wpg - 1) / wpg, full_blocks4 * 4 / wpg));
+    return {g, g * wpg, true};  // never past the full grid
 }
 
 // The 64-bit remap runs with keys in scalar registers only.

@@ -602,7 +602,7 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 // Byte table of the BT instance: entry h (a 16-bit hash) holds fb * part for the
 // part floor(((h * bt_m) >> 16) / SPLIT) of a node of bt_m keys, so one LDS byte
 // read replaces the remap and the increment table lookup.  65536 entries in the
-// workgroup's static LDS (at most two 512-thread workgroups per CU).
+// workgroup's static LDS (two workgroups per CU).
 template <bool BT>
 __device__ __forceinline__ const uint8_t *bt_table() {
     if constexpr (BT) {
@@ -772,9 +772,16 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
 // RB (rotation leaves of at most 16 keys): packed masks and the candidate test
 // batched through LDS, with registers held to five waves per SIMD.  BT
 // (aggregation levels, 32-bit counters, 16-bit remap): the byte table for nodes of
-// bt_m keys, workgroups of 512 threads, two per CU.
+// bt_m keys, workgroups of kBtThreads threads, two per CU (registers held to the
+// kBtThreads / 128 waves per SIMD that two such workgroups need).  Measured at C2:
+// 768 threads (six waves per SIMD) beat 512 (four) by 3%; 640 (ten waves, which
+// do not split evenly over four SIMDs) ran 17% slower than 512.
+#ifndef RS_BT_THREADS
+#define RS_BT_THREADS 768
+#endif
+constexpr uint32_t kBtThreads = RS_BT_THREADS;
 template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
-__global__ __launch_bounds__(BT ? 512 : 256, RB ? 5 : (BT ? 2 : 1)) void k_search(const uint64_t *__restrict__ keys,
+__global__ __launch_bounds__(BT ? kBtThreads : 256, RB ? 5 : (BT ? kBtThreads / 128 : 1)) void k_search(const uint64_t *__restrict__ keys,
                                                                                   SearchArgs a) {
     static_assert(!BT || ((KIND == kUpper || KIND == kLower) && LUT && !WIDE && !R64 && !RB),
                   "the byte table serves 32-bit aggregation counters over the 16-bit remap");
