@@ -266,7 +266,8 @@ using namespace rsa;
 // sk / skK: keys in scalar registers for every kind / kind K,
 // gK: 64*u-candidate groups per claimed chunk of kind K, hK: helper cap of kind K
 // (0: none), bpc: workgroups of 4 waves per CU, bt: byte table for aggregation
-// levels of mostly full-size nodes, ubt: candidates per lane there).  Fanout-2
+// levels of mostly full-size nodes, ubt: candidates per lane there, btl: byte
+// table of positions for rotation leaves).  Fanout-2
 // nodes need ~40 candidates, so they run one per lane.
 struct Tune {
     int u[kNumKinds] = {1, 2, 2, 4, 4};
@@ -278,7 +279,8 @@ struct Tune {
     int help_cap[kNumKinds] = {2, 0, 0, 0, 0};
     int blocks_per_cu = 8;
     int bt = 1;      // byte table for the full-size nodes of aggregation levels
-    int u_bt = 4;    // candidates in flight per lane in byte-table launches
+    int u_bt = 4;    // candidates in flight per lane in those launches
+    int bt_leaf = 1; // byte table of positions for rotation leaves
 };
 
 static Tune parse_tune(const char *env, Tune t = Tune()) {
@@ -301,6 +303,7 @@ static Tune parse_tune(const char *env, Tune t = Tune()) {
             else if (k == "lut") t.lut = v != 0;
             else if (k == "bt") t.bt = v != 0;
             else if (k == "ubt" && uv) t.u_bt = v;
+            else if (k == "btl") t.bt_leaf = v != 0;
             else if (k == "sk")
                 for (int &x : t.sk) x = v != 0;
             else if (k.size() == 3 && k[0] == 's' && k[1] == 'k' && k[2] >= '0' && k[2] < '0' + (int)kNumKinds)
@@ -327,6 +330,7 @@ static std::string format_tune(const Tune &t) {
     add("lut", t.lut);
     add("bt", t.bt);
     add("ubt", t.u_bt);
+    add("btl", t.bt_leaf);
     add("bpc", t.blocks_per_cu);
     return s;
 }
@@ -463,8 +467,15 @@ static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
-    } else if (KIND == kLeafRot && a.leaf <= 16) {
-        k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
+    } else if constexpr (KIND == kLeafRot) {
+        if (a.bt_m && !R64 && a.leaf <= 16)
+            k_search<KIND, U, false, false, SK, false, true, true><<<blocks, kBtThreads, 0, s>>>(keys, a);
+        else if (a.bt_m && !R64)
+            k_search<KIND, U, false, false, SK, false, false, true><<<blocks, kBtThreads, 0, s>>>(keys, a);
+        else if (a.leaf <= 16)
+            k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
+        else
+            k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     } else {
         k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     }
@@ -493,17 +504,19 @@ static void launch_sk(hipStream_t s, int u, uint32_t kind, const uint64_t *keys,
 }
 
 // Grid of one search level from its size in 4-wave workgroups: the byte-table
-// instance (aggregation levels with 32-bit counters and the 16-bit remap whose
-// nodes are mostly of the full size) runs kBtThreads-thread workgroups, two per CU,
-// since its 64 KiB table is per workgroup.
+// instance (aggregation levels with 32-bit counters whose nodes are mostly of the
+// full size, and rotation leaves, both over the 16-bit remap) runs
+// kBtThreads-thread workgroups, two per CU, since its 64 KiB table is per
+// workgroup.
 struct SearchShape {
     uint32_t grid, waves;
     bool bt;
 };
 static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool lut, bool r64, bool mostly_full,
                                 uint32_t blocks4, uint32_t full_blocks4) {
-    if (!(tn.bt && (kind == kUpper || kind == kLower) && lut && !wide && !r64 && mostly_full))
-        return {blocks4, blocks4 * 4, false};
+    const bool aggr = tn.bt && (kind == kUpper || kind == kLower) && lut && !wide && mostly_full;
+    const bool leaf = tn.bt_leaf && kind == kLeafRot;
+    if (!((aggr || leaf) && !r64)) return {blocks4, blocks4 * 4, false};
     const uint32_t wpg = kBtThreads / 64;  // waves per workgroup
     const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 * 4 + wpg - 1) / wpg, full_blocks4 * 4 / wpg));
     return {g, g * wpg, true};  // never past the full grid
@@ -512,7 +525,7 @@ static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool l
 // The 64-bit remap runs with keys in scalar registers only.
 static void launch_search(hipStream_t s, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
                           bool wide, bool lut, uint32_t blocks, size_t lds, bool r64) {
-    const int u = a.bt_m ? tn.u_bt : tn.u[kind];
+    const int u = a.bt_m && (kind == kUpper || kind == kLower) ? tn.u_bt : tn.u[kind];
     if (r64) launch_sk<true, true>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else if (tn.sk[kind]) launch_sk<true, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else launch_sk<false, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
@@ -774,7 +787,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             if (!lut) lds = 0;
         }
         const SearchShape sh = search_shape(tn, kind, wide, lut, r64, 2 * cen.full[l] >= cnt, lblocks, blocks);
-        a.bt_m = sh.bt ? a.lut_entries : 0;
+        a.bt_m = sh.bt ? (kind == kLeafRot ? P.leaf : a.lut_entries) : 0;
         a.n_waves = sh.waves;
         if (sh.waves > n_waves) throw BuildError(RS_ERR_INTERNAL, "search launch beyond the running table");
         HIP_CHECK(hipMemsetAsync(running, 0xFF, (size_t)sh.waves * 4, s));

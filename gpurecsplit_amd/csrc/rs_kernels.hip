@@ -599,10 +599,19 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
     return kNone;
 }
 
-// Byte table of the BT instance: entry h (a 16-bit hash) holds fb * part for the
-// part floor(((h * bt_m) >> 16) / SPLIT) of a node of bt_m keys, so one LDS byte
-// read replaces the remap and the increment table lookup.  65536 entries in the
-// workgroup's static LDS (two workgroups per CU).
+// Byte table of the BT instance, 65536 entries in the workgroup's static LDS
+// (two workgroups of kBtThreads threads per CU).  Aggregation levels: entry h (a
+// 16-bit hash) holds fb * part for the part floor(((h * bt_m) >> 16) / SPLIT) of a
+// node of bt_m keys, so one LDS byte read replaces the remap and the increment
+// table lookup.  Rotation leaves: entry h holds the position (h * L) >> 16.
+// Workgroup size measured at C2: 768 threads (six waves per SIMD) beat 512 (four)
+// by 3%; 640 (ten waves, which do not split evenly over four SIMDs) ran 17% slower
+// than 512.
+#ifndef RS_BT_THREADS
+#define RS_BT_THREADS 768
+#endif
+constexpr uint32_t kBtThreads = RS_BT_THREADS;
+
 template <bool BT>
 __device__ __forceinline__ const uint8_t *bt_table() {
     if constexpr (BT) {
@@ -634,9 +643,11 @@ __device__ __forceinline__ uint64_t group_aggr_bt(const uint64_t *__restrict__ k
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
 // The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
-template <int U, bool SK, bool R64, bool RB>
+template <int U, bool SK, bool R64, bool RB, bool BT = false>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
                                                    uint32_t nr, uint64_t u0) {
+    // BT: position (h16 * L) >> 16 of every 16-bit hash read from the byte table
+    const uint8_t *bt = bt_table<BT>();
     const uint64_t S = c_start_seed[kSeedLeaf];
     const uint32_t full = (uint32_t)((1ull << L) - 1);
     const Seeds<U> sd(u0 * L + S, L);
@@ -651,6 +662,9 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
         if constexpr (R64) {
             each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { ml[u] |= 1u << r; });
             each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mr[u] |= 1u << r; });
+        } else if constexpr (BT) {
+            each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << bt[h]; });
+            each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << bt[h]; });
         } else {
             each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { ml[u] |= 1u << (__umul24(h, L) >> 16); });
             each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << (__umul24(h, L) >> 16); });
@@ -685,13 +699,25 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
         if constexpr (R64) {
             each_r64<U>(kp, 0, nl, sd, L, [&](int u, uint32_t r) { mk[u] |= 1u << r; });
             each_r64<U>(kp, nl, L, sd, L, [&](int u, uint32_t r) { mk[u] |= 1u << (r + 16); });
+        } else if constexpr (BT) {
+            uint32_t mr[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) mr[u] = 0;
+            each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { mk[u] |= 1u << bt[h]; });
+            each_hash<U, SK>(kp, nl, L, sd, [&](int u, uint32_t h) { mr[u] |= 1u << bt[h]; });
+#pragma unroll
+            for (int u = 0; u < U; u++) mk[u] |= mr[u] << 16;
         } else {
             each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, L) >> 16); });
             each_hash<U, SK>(kp, nl, L, sd,
                              [&](int u, uint32_t h) { mk[u] |= 1u << ((__umul24(h, L) + (16u << 16)) >> 16); });
         }
-        __shared__ uint32_t s_cand[4][2][U * 64];  // one list per wave of the workgroup
-        uint32_t *cm = s_cand[threadIdx.x >> 6][0], *cu = s_cand[threadIdx.x >> 6][1];
+        // one list per wave of the workgroup: masks, and unit indexes (< 256) as bytes
+        constexpr int kWaves = (BT ? kBtThreads : 256) / 64;
+        __shared__ uint32_t s_cand_m[kWaves][U * 64];
+        __shared__ uint8_t s_cand_u[kWaves][U * 64];
+        uint32_t *cm = s_cand_m[threadIdx.x >> 6];
+        uint8_t *cu = s_cand_u[threadIdx.x >> 6];
         uint32_t count = 0;
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -701,7 +727,7 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
                 __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             if (ok) {
                 cm[count + below] = mk[u];
-                cu[count + below] = (uint32_t)(u * 64) + lane;
+                cu[count + below] = (uint8_t)(u * 64 + lane);
             }
             count += (uint32_t)__popcll(bal);
         }
@@ -773,18 +799,12 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
 // batched through LDS, with registers held to five waves per SIMD.  BT
 // (aggregation levels, 32-bit counters, 16-bit remap): the byte table for nodes of
 // bt_m keys, workgroups of kBtThreads threads, two per CU (registers held to the
-// kBtThreads / 128 waves per SIMD that two such workgroups need).  Measured at C2:
-// 768 threads (six waves per SIMD) beat 512 (four) by 3%; 640 (ten waves, which
-// do not split evenly over four SIMDs) ran 17% slower than 512.
-#ifndef RS_BT_THREADS
-#define RS_BT_THREADS 768
-#endif
-constexpr uint32_t kBtThreads = RS_BT_THREADS;
+// kBtThreads / 128 waves per SIMD that two such workgroups need).
 template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
-__global__ __launch_bounds__(BT ? kBtThreads : 256, RB ? 5 : (BT ? kBtThreads / 128 : 1)) void k_search(const uint64_t *__restrict__ keys,
+__global__ __launch_bounds__(BT ? kBtThreads : 256, BT ? kBtThreads / 128 : (RB ? 5 : 1)) void k_search(const uint64_t *__restrict__ keys,
                                                                                   SearchArgs a) {
-    static_assert(!BT || ((KIND == kUpper || KIND == kLower) && LUT && !WIDE && !R64 && !RB),
-                  "the byte table serves 32-bit aggregation counters over the 16-bit remap");
+    static_assert(!BT || ((((KIND == kUpper || KIND == kLower) && LUT && !WIDE && !RB) || KIND == kLeafRot) && !R64),
+                  "the byte table serves 32-bit aggregation counters and rotation leaves over the 16-bit remap");
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -821,6 +841,17 @@ __global__ __launch_bounds__(BT ? kBtThreads : 256, RB ? 5 : (BT ? kBtThreads / 
                 }
                 bt_w[w] = v;
             }
+        }
+        __syncthreads();
+    }
+    if constexpr (BT && KIND == kLeafRot) {
+        uint32_t *bt_w = (uint32_t *)bt_table<BT>();
+        const uint32_t L = a.bt_m;
+        for (uint32_t w = threadIdx.x; w < 65536 / 4; w += blockDim.x) {
+            uint32_t v = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) v |= (((w * 4 + j) * L) >> 16) << (8 * j);
+            bt_w[w] = v;
         }
         __syncthreads();
     }
@@ -1021,7 +1052,7 @@ __global__ __launch_bounds__(BT ? kBtThreads : 256, RB ? 5 : (BT ? kBtThreads / 
             else if (KIND == kUpper || KIND == kLower)
                 hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
                                                         u0);
-            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64, RB>(kp, m, aux0, aux1, u0);
+            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64, RB, BT>(kp, m, aux0, aux1, u0);
             else hit = group_leaf_brute<U, SK, R64>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
