@@ -266,9 +266,9 @@ using namespace rsa;
 // sk / skK: keys in scalar registers for every kind / kind K,
 // gK: 64*u-candidate groups per claimed chunk of kind K, hK: helper cap of kind K
 // (0: none), bpc: workgroups of 4 waves per CU, bt: byte table for aggregation
-// levels of mostly full-size nodes, ubt: candidates per lane there, btl: byte
-// table of positions for rotation leaves).  Fanout-2
-// nodes need ~40 candidates, so they run one per lane.
+// levels of mostly full-size nodes, ubt: candidates per lane there, btw / ubtw: the
+// same for 64-bit counters, btl: byte table of positions for rotation leaves).
+// Fanout-2 nodes need ~40 candidates, so they run one per lane.
 struct Tune {
     int u[kNumKinds] = {1, 2, 2, 4, 4};
     int sk[kNumKinds] = {1, 1, 1, 1, 1};  // keys in scalar registers
@@ -281,6 +281,8 @@ struct Tune {
     int bt = 1;      // byte table for the full-size nodes of aggregation levels
     int u_bt = 4;    // candidates in flight per lane in those launches
     int bt_leaf = 1; // byte table of positions for rotation leaves
+    int bt_wide = 1; // byte table also for 64-bit counters (leaf sizes above 18)
+    int u_bt_wide = 2; // candidates per lane there (four spill at 80 registers)
 };
 
 static Tune parse_tune(const char *env, Tune t = Tune()) {
@@ -304,6 +306,8 @@ static Tune parse_tune(const char *env, Tune t = Tune()) {
             else if (k == "bt") t.bt = v != 0;
             else if (k == "ubt" && uv) t.u_bt = v;
             else if (k == "btl") t.bt_leaf = v != 0;
+            else if (k == "btw") t.bt_wide = v != 0;
+            else if (k == "ubtw" && uv) t.u_bt_wide = v;
             else if (k == "sk")
                 for (int &x : t.sk) x = v != 0;
             else if (k.size() == 3 && k[0] == 's' && k[1] == 'k' && k[2] >= '0' && k[2] < '0' + (int)kNumKinds)
@@ -331,6 +335,8 @@ static std::string format_tune(const Tune &t) {
     add("bt", t.bt);
     add("ubt", t.u_bt);
     add("btl", t.bt_leaf);
+    add("btw", t.bt_wide);
+    add("ubtw", t.u_bt_wide);
     add("bpc", t.blocks_per_cu);
     return s;
 }
@@ -461,7 +467,9 @@ template <int KIND, int U, bool SK, bool R64>
 static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a, bool wide, bool lut, uint32_t blocks,
                         size_t lds) {
     if constexpr (KIND == kUpper || KIND == kLower) {
-        if (a.bt_m && !wide && lut && !R64)
+        if (a.bt_m && wide && lut && !R64)
+            k_search<KIND, U, true, true, SK, false, false, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
+        else if (a.bt_m && lut && !R64)
             k_search<KIND, U, false, true, SK, false, false, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
         else if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
@@ -514,7 +522,7 @@ struct SearchShape {
 };
 static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool lut, bool r64, bool mostly_full,
                                 uint32_t blocks4, uint32_t full_blocks4) {
-    const bool aggr = tn.bt && (kind == kUpper || kind == kLower) && lut && !wide && mostly_full;
+    const bool aggr = tn.bt && (kind == kUpper || kind == kLower) && lut && (!wide || tn.bt_wide) && mostly_full;
     const bool leaf = tn.bt_leaf && kind == kLeafRot;
     if (!((aggr || leaf) && !r64)) return {blocks4, blocks4 * 4, false};
     const uint32_t wpg = kBtThreads / 64;  // waves per workgroup
@@ -525,7 +533,7 @@ static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool l
 // The 64-bit remap runs with keys in scalar registers only.
 static void launch_search(hipStream_t s, const Tune &tn, uint32_t kind, const uint64_t *keys, const SearchArgs &a,
                           bool wide, bool lut, uint32_t blocks, size_t lds, bool r64) {
-    const int u = a.bt_m && (kind == kUpper || kind == kLower) ? tn.u_bt : tn.u[kind];
+    const int u = a.bt_m && (kind == kUpper || kind == kLower) ? (wide ? tn.u_bt_wide : tn.u_bt) : tn.u[kind];
     if (r64) launch_sk<true, true>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else if (tn.sk[kind]) launch_sk<true, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
     else launch_sk<false, false>(s, u, kind, keys, a, wide, lut, blocks, lds);
@@ -1150,6 +1158,7 @@ static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags
                 t.groups[k] = g;
             }
             t.u_bt = u;
+            t.u_bt_wide = u;
             const rs_build_stats st = timed(t);
             rep += std::string(first ? "" : ",") + "{\"u\":" + std::to_string(u) + ",\"g\":" + std::to_string(g) +
                    ",\"search_ms\":[";
@@ -1159,7 +1168,7 @@ static std::string autotune(rs_ctx *c, uint32_t leaf, uint64_t bucket, int flags
                     best_ms[k] = st.search_ms[k];
                     best.u[k] = u;
                     best.groups[k] = g;
-                    if (k == kLower) best.u_bt = u;  // byte-table launches (lower level dominates)
+                    if (k == kLower) best.u_bt = best.u_bt_wide = u;  // byte-table launches (lower level dominates)
                 }
             }
             rep += "]}";

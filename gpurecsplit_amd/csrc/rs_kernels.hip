@@ -623,14 +623,15 @@ __device__ __forceinline__ const uint8_t *bt_table() {
 }
 
 // Aggregation node of the byte table's size: the same predicate as group_aggr,
-// the increment of every key 1 << bt[h16] (h16 < 2^16, 32-bit counters).
-template <int U, bool SK>
+// the increment of every key 1 << bt[h16] (h16 < 2^16; 64-bit counters for WIDE).
+template <int U, bool SK, bool WIDE>
 __device__ __forceinline__ uint64_t group_aggr_bt(const uint64_t *__restrict__ kp, uint32_t m, uint64_t sconst,
                                                   const uint8_t *bt, uint64_t T, uint64_t cmask, uint64_t u0) {
-    uint32_t cnt[U];
+    using C = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    C cnt[U];
 #pragma unroll
     for (int u = 0; u < U; u++) cnt[u] = 0;
-    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += 1u << bt[h]; });
+    each_hash<U, SK>(kp, 0, m, Seeds<U>(u0 + sconst, 1), [&](int u, uint32_t h) { cnt[u] += (C)1 << bt[h]; });
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const unsigned long long msk = __ballot(((uint64_t)cnt[u] & cmask) == T);
@@ -803,8 +804,8 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
 template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
 __global__ __launch_bounds__(BT ? kBtThreads : 256, BT ? kBtThreads / 128 : (RB ? 5 : 1)) void k_search(const uint64_t *__restrict__ keys,
                                                                                   SearchArgs a) {
-    static_assert(!BT || ((((KIND == kUpper || KIND == kLower) && LUT && !WIDE && !RB) || KIND == kLeafRot) && !R64),
-                  "the byte table serves 32-bit aggregation counters and rotation leaves over the 16-bit remap");
+    static_assert(!BT || ((((KIND == kUpper || KIND == kLower) && LUT && !RB) || KIND == kLeafRot) && !R64),
+                  "the byte table serves aggregation levels and rotation leaves over the 16-bit remap");
     extern __shared__ uint32_t s_lut[];
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1048,7 +1049,7 @@ __global__ __launch_bounds__(BT ? kBtThreads : 256, BT ? kBtThreads / 128 : (RB 
             work += (uint64_t)U * 64 * m;
             if (KIND == kHigher) hit = group_higher<U, SK, R64>(kp, m, sconst, aux0, aux1, u0);
             else if ((KIND == kUpper || KIND == kLower) && BT && m == a.bt_m)
-                hit = group_aggr_bt<U, SK>(kp, m, sconst, bt_table<BT>(), T, cmask, u0);
+                hit = group_aggr_bt<U, SK, WIDE>(kp, m, sconst, bt_table<BT>(), T, cmask, u0);
             else if (KIND == kUpper || KIND == kLower)
                 hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
                                                         u0);
