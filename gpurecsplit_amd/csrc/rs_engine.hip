@@ -270,7 +270,11 @@ using namespace rsa;
 // same for 64-bit counters, btl: byte table of positions for rotation leaves).
 // Fanout-2 nodes need ~40 candidates, so they run one per lane.
 struct Tune {
-    int u[kNumKinds] = {1, 2, 2, 4, 4};
+    // upper aggregation one per lane: its increment-table launches are the levels
+    // whose nodes are mostly below the full size (whole buckets of about b keys),
+    // which need few candidates (4096-set batch at l = 12: 20.2 -> 15.8 ms); the
+    // full-size levels run the byte table with u_bt
+    int u[kNumKinds] = {1, 1, 2, 4, 4};
     int sk[kNumKinds] = {1, 1, 1, 1, 1};  // keys in scalar registers
     int lut = 1;
     int groups[kNumKinds] = {1, 1, 2, 4, 4};
