@@ -481,9 +481,9 @@ static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     } else if constexpr (KIND == kLeafRot) {
         if (a.bt_m && !R64 && a.leaf <= 16)
-            k_search<KIND, U, false, false, SK, false, true, true><<<blocks, kBtThreads, 0, s>>>(keys, a);
+            k_search<KIND, U, false, false, SK, false, true, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
         else if (a.bt_m && !R64)
-            k_search<KIND, U, false, false, SK, false, false, true><<<blocks, kBtThreads, 0, s>>>(keys, a);
+            k_search<KIND, U, false, false, SK, false, false, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
         else if (a.leaf <= 16)
             k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
         else
@@ -529,7 +529,7 @@ static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool l
     const bool aggr = tn.bt && (kind == kUpper || kind == kLower) && lut && (!wide || tn.bt_wide) && mostly_full;
     const bool leaf = tn.bt_leaf && kind == kLeafRot;
     if (!((aggr || leaf) && !r64)) return {blocks4, blocks4 * 4, false};
-    const uint32_t wpg = kBtThreads / 64;  // waves per workgroup
+    const uint32_t wpg = bt_threads((int)kind) / 64;  // waves per workgroup
     const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 * 4 + wpg - 1) / wpg, full_blocks4 * 4 / wpg));
     return {g, g * wpg, true};  // never past the full grid
 }

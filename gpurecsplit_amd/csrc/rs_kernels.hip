@@ -611,6 +611,13 @@ __device__ __forceinline__ uint64_t group_aggr(const uint64_t *__restrict__ kp, 
 #define RS_BT_THREADS 768
 #endif
 constexpr uint32_t kBtThreads = RS_BT_THREADS;
+#ifndef RS_BT_THREADS_LEAF
+#define RS_BT_THREADS_LEAF 768
+#endif
+// the rotation-leaf instances: 768 as well (512, four waves per SIMD without the
+// spills of the 80-register bound, measured slower: C3 leaves 319 -> 351 ms)
+constexpr uint32_t kBtThreadsLeaf = RS_BT_THREADS_LEAF;
+__host__ __device__ constexpr uint32_t bt_threads(int kind) { return kind == kLeafRot ? kBtThreadsLeaf : kBtThreads; }
 
 template <bool BT>
 __device__ __forceinline__ const uint8_t *bt_table() {
@@ -714,7 +721,7 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
                              [&](int u, uint32_t h) { mk[u] |= 1u << ((__umul24(h, L) + (16u << 16)) >> 16); });
         }
         // one list per wave of the workgroup: masks, and unit indexes (< 256) as bytes
-        constexpr int kWaves = (BT ? kBtThreads : 256) / 64;
+        constexpr int kWaves = (BT ? kBtThreadsLeaf : 256) / 64;
         __shared__ uint32_t s_cand_m[kWaves][U * 64];
         __shared__ uint8_t s_cand_u[kWaves][U * 64];
         uint32_t *cm = s_cand_m[threadIdx.x >> 6];
@@ -802,7 +809,7 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
 // bt_m keys, workgroups of kBtThreads threads, two per CU (registers held to the
 // kBtThreads / 128 waves per SIMD that two such workgroups need).
 template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
-__global__ __launch_bounds__(BT ? kBtThreads : 256, BT ? kBtThreads / 128 : (RB ? 5 : 1)) void k_search(const uint64_t *__restrict__ keys,
+__global__ __launch_bounds__(BT ? bt_threads(KIND) : 256, BT ? bt_threads(KIND) / 128 : (RB ? 5 : 1)) void k_search(const uint64_t *__restrict__ keys,
                                                                                   SearchArgs a) {
     static_assert(!BT || ((((KIND == kUpper || KIND == kLower) && LUT && !RB) || KIND == kLeafRot) && !R64),
                   "the byte table serves aggregation levels and rotation leaves over the 16-bit remap");
