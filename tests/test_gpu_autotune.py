@@ -34,3 +34,19 @@ def test_autotune_keeps_bytes(gra, leaf, bucket):
     t = other.get_tune(leaf)
     assert "u2=1" in t and "bpc=4" in t
     assert gra.GPURecSplit(keys, bucket, leaf_size=leaf, ctx=other).serialize() == got
+
+
+@pytest.mark.parametrize("leaf,bucket,n", [(8, 100, 20000), (12, 100, 20000), (16, 2000, 30000),
+                                           (18, 50, 6000), (20, 900, 1800), (24, 24, 480)])
+def test_kernels_without_byte_tables_keep_bytes(gra, leaf, bucket, n):
+    """The increment-table and remap-multiply kernels that the byte tables replace
+    by default (tuning keys bt, btw, btl off) build the oracle's bytes, and so do
+    the byte-table kernels at two candidates per lane."""
+    keys = splitmix_keys(leaf * 7 + n, n)
+    exp = OracleMPHF.build(keys, leaf, bucket, threads=8).serialize()
+    for tune in ("bt=0,btw=0,btl=0", "ubt=2,ubtw=2,u3=2"):
+        ctx = gra.Context(0)
+        ctx.set_tune(tune, leaf)
+        print(f"leaf={leaf} bucket={bucket} n={n} tune={tune}", flush=True)
+        assert gra.GPURecSplit(keys, bucket, leaf_size=leaf, ctx=ctx).serialize() == exp
+        ctx.close()
