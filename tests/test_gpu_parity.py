@@ -60,6 +60,28 @@ def test_bytes_match_oracle(torch_cuda, leaf, bucket, n):
     _parity(keys, leaf, bucket)
 
 
+def _random_shapes(count=14, seed=20261203):
+    """Deterministic draws of (leaf, bucket, n, rotate): leaf sizes 2..18, buckets
+    from a few keys to 2000, up to 20,000 keys, both leaf modes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        leaf = int(rng.integers(2, 19))
+        bucket = int(np.exp(rng.uniform(np.log(2), np.log(2000))))
+        n = int(rng.integers(1, 20001))
+        out.append((leaf, max(bucket, 1), n, bool(rng.integers(0, 4) > 0)))
+    return out
+
+
+@pytest.mark.parametrize("leaf,bucket,n,rotate", _random_shapes())
+def test_random_shapes_match_oracle(torch_cuda, leaf, bucket, n, rotate):
+    """Shapes nobody picked: every launch kind, grid size and kernel instance (byte
+    tables where a level's nodes are mostly full-size, increment tables elsewhere)
+    has to give the oracle's bytes."""
+    keys = splitmix_keys(leaf * 31 + bucket * 7 + n, n)
+    _parity(keys, leaf, bucket, rotate=rotate)
+
+
 @pytest.mark.parametrize("leaf,bucket,n", [(2, 7, 2000), (5, 10, 5000), (8, 100, 20000), (11, 75, 3000)])
 def test_bytes_match_oracle_bruteforce_leaves(torch_cuda, leaf, bucket, n):
     keys = splitmix_keys(leaf + 5, n)
