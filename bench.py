@@ -389,7 +389,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(buckets shard by contiguous ranges across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(buckets would shard by contiguous ranges across GPUs)", file=sys.stderr)
         return 2
 
     import torch
