@@ -277,7 +277,12 @@ struct Tune {
     int u[kNumKinds] = {1, 1, 2, 4, 4};
     int sk[kNumKinds] = {1, 1, 1, 1, 1};  // keys in scalar registers
     int lut = 1;
-    int groups[kNumKinds] = {1, 1, 2, 4, 4};
+    // groups per claimed chunk: a wave checks the node's result between groups, so
+    // long chunks cost no overshoot on nodes a wave searches alone and save claims
+    // (leaves 4 -> 16 groups: C2 leaves 68.0 -> 64.2 ms, C3 319 -> 309 ms; lower
+    // aggregation 2 -> 8: 0.6% at C2 but up to 25% slower on small builds' levels,
+    // whose few nodes are shared by helpers)
+    int groups[kNumKinds] = {1, 1, 2, 16, 16};
     // helpers join a node only while it has fewer waves than this (0: no cap);
     // fanout-2 nodes need ~40 candidates, so extra waves mostly overshoot
     int help_cap[kNumKinds] = {2, 0, 0, 0, 0};
