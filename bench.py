@@ -53,7 +53,7 @@ KINDS = ["fanout2", "upper_aggr", "lower_aggr", "leaf_rotation", "leaf_bruteforc
 # two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0: the factors of
 # a 256 MiB copy probe); the profile summaries under profiles/ give the commands.
 # Most of it is register spills written around chunk claims, not key traffic.
-TRAFFIC_BYTES = {"lower_aggr": 15.92e9, "upper_aggr": 10.51e9, "leaf_rotation": 47.83e9}
+TRAFFIC_BYTES = {"lower_aggr": 15.90e9, "upper_aggr": 10.40e9, "leaf_rotation": 12.20e9}
 
 
 def log(msg: str):
