@@ -530,11 +530,19 @@ struct SearchShape {
     bool bt;
 };
 static SearchShape search_shape(const Tune &tn, uint32_t kind, bool wide, bool lut, bool r64, bool mostly_full,
-                                uint32_t blocks4, uint32_t full_blocks4) {
+                                uint32_t blocks4, uint32_t full_blocks4, uint32_t num_cus) {
     const bool aggr = tn.bt && (kind == kUpper || kind == kLower) && lut && (!wide || tn.bt_wide) && mostly_full;
     const bool leaf = tn.bt_leaf && kind == kLeafRot;
-    if (!((aggr || leaf) && !r64)) return {blocks4, blocks4 * 4, false};
     const uint32_t wpg = bt_threads((int)kind) / 64;  // waves per workgroup
+    // A leaf level of fewer waves than half a byte-table workgroup per CU would
+    // put its waves on a few CUs (sharing their SIMDs) while others idle; it runs
+    // the 4-wave workgroups instead.  Measured rotation levels: 828 waves (10K keys,
+    // l = 16) 0.84 -> 0.67 ms, 984 waves (100K keys, l = 12) 0.41 -> 0.29 ms; at
+    // 2612 waves (100K keys, l = 16) the byte table is faster (2.32 against 2.53 ms).
+    // Small aggregation levels keep the byte table, whose instruction savings
+    // outweigh that (10K keys at l = 16: upper 1.68 ms with it, 2.54 ms without).
+    const bool fills = (uint64_t)blocks4 * 4 * 2 >= (uint64_t)num_cus * wpg;
+    if (!((aggr || (leaf && fills)) && !r64)) return {blocks4, blocks4 * 4, false};
     const uint32_t g = std::max<uint32_t>(1, std::min((blocks4 * 4 + wpg - 1) / wpg, full_blocks4 * 4 / wpg));
     return {g, g * wpg, true};  // never past the full grid
 }
@@ -803,7 +811,8 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
-        const SearchShape sh = search_shape(tn, kind, wide, lut, r64, 2 * cen.full[l] >= cnt, lblocks, blocks);
+        const SearchShape sh =
+            search_shape(tn, kind, wide, lut, r64, 2 * cen.full[l] >= cnt, lblocks, blocks, (uint32_t)c->num_cus);
         a.bt_m = sh.bt ? (kind == kLeafRot ? P.leaf : a.lut_entries) : 0;
         a.n_waves = sh.waves;
         if (sh.waves > n_waves) throw BuildError(RS_ERR_INTERNAL, "search launch beyond the running table");

@@ -334,7 +334,8 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
             lut = tn.lut != 0 && lds <= 32768;
             if (!lut) lds = 0;
         }
-        const SearchShape sh = search_shape(tn, kind, wide, lut, r64, p->mostly_full[li] != 0, blocks, p->blocks);
+        const SearchShape sh =
+            search_shape(tn, kind, wide, lut, r64, p->mostly_full[li] != 0, blocks, p->blocks, (uint32_t)p->num_cus);
         a.bt_m = sh.bt ? (kind == kLeafRot ? P.leaf : a.lut_entries) : 0;
         a.n_waves = sh.waves;
         k_level_init<<<likely ? fill_grid : 4, 256, 0, s>>>(p->state, p->running, p->ctl, sh.waves, p->list_count,
