@@ -170,7 +170,8 @@ int rs_ctx_set_tune(rs_ctx *ctx, uint32_t leaf_size, const char *tune);
 int rs_ctx_get_tune(rs_ctx *ctx, uint32_t leaf_size, char *buf, size_t cap);
 
 /* Device memory the context holds for reuse by later builds (grow-only buffers),
- * and their release.  No reference counterpart. */
+ * and their release (which also frees the pinned host staging of batch results).
+ * No reference counterpart. */
 uint64_t rs_ctx_device_bytes(rs_ctx *ctx);
 int rs_ctx_release(rs_ctx *ctx);
 
