@@ -669,14 +669,16 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     std::vector<uint64_t> nbs(S), bbase(S + 1);
     std::vector<KeyTile> tiles;
     uint64_t NB = 0, max_nb = 0;
+    // larger tiles when there are keys for two per CU: fewer global atomics per key
+    const uint64_t tile_keys = set_off[S] - set_off[0] >= 8 * kTileKeys * (uint64_t)c->num_cus ? 4 * kTileKeys : kTileKeys;
     for (size_t i = 0; i < S; i++) {
         const uint64_t n = set_off[i + 1] - set_off[i];
         nbs[i] = std::max<uint64_t>(1, (n + bucket_size - 1) / bucket_size);
         bbase[i] = NB;
         NB += nbs[i];
         max_nb = std::max(max_nb, nbs[i]);
-        for (uint64_t k = 0; k < n; k += kTileKeys)
-            tiles.push_back(KeyTile{set_keys[i], k, std::min(k + kTileKeys, n), nbs[i], bbase[i]});
+        for (uint64_t k = 0; k < n; k += tile_keys)
+            tiles.push_back(KeyTile{set_keys[i], k, std::min(k + tile_keys, n), nbs[i], bbase[i]});
     }
     bbase[S] = NB;
     const uint64_t n_total = set_off[S] - set_off[0];
@@ -705,7 +707,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     HIP_CHECK(hipMemsetAsync(counts, 0, NB * 4, s));
     HIP_CHECK(hipMemsetAsync(size_hist, 0, (kMaxBucket + 1) * 4, s));
     HIP_CHECK(hipMemsetAsync(err, 0, 16, s));
-    const uint32_t priv_max = (uint32_t)std::min<uint64_t>(max_nb, 8192);
+    const uint32_t priv_max = (uint32_t)std::min<uint64_t>(max_nb, 16384);
     k_bucket_hist<<<tiles.size(), 256, (size_t)priv_max * 4, s>>>(d_tiles, counts, priv_max);
     HIP_CHECK(hipGetLastError());
     k_bucket_meta<<<(uint32_t)std::min<uint64_t>((NB + 255) / 256, (uint64_t)c->num_cus * 4), 256, 0, s>>>(
@@ -740,7 +742,7 @@ static std::vector<rs_mphf *> build_sets(rs_ctx *c, const std::vector<const uint
     uint64_t *sorted = c->sorted.get<uint64_t>(n_total + 8);  // +8: look-ahead loads of the search
     uint32_t *cursor = c->cursor.get<uint32_t>(NB);
     HIP_CHECK(hipMemsetAsync(cursor, 0, NB * 4, s));
-    k_scatter<<<tiles.size(), 256, 0, s>>>(d_tiles, acc, cursor, sorted);
+    k_scatter<<<tiles.size(), 256, (size_t)priv_max * 4, s>>>(d_tiles, acc, cursor, sorted, priv_max);
     HIP_CHECK(hipGetLastError());
     uint32_t sort_p = 1;
     while (sort_p < st.max_bucket_size) sort_p <<= 1;

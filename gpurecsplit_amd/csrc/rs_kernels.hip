@@ -165,10 +165,35 @@ __global__ __launch_bounds__(256) void k_bucket_meta(const uint32_t *__restrict_
         if (s_hist[i]) atomicAdd(&size_hist[i], s_hist[i]);
 }
 
+// Each key's `second` to a free slot of its bucket's range (any order: the sort
+// that follows makes it canonical).  Tiles of sets with at most `priv_max` buckets
+// count their keys per bucket in LDS, reserve each touched bucket's slots with one
+// global atomic and place their keys by LDS atomics at set-relative positions.
+// Larger sets take one global atomic per key.
 __global__ __launch_bounds__(256) void k_scatter(const KeyTile *__restrict__ tiles, const uint64_t *__restrict__ bucket_acc,
-                                                 uint32_t *__restrict__ cursor, uint64_t *__restrict__ sorted) {
+                                                 uint32_t *__restrict__ cursor, uint64_t *__restrict__ sorted,
+                                                 uint32_t priv_max) {
+    extern __shared__ uint32_t s_pos[];  // priv_max entries
     const KeyTile t = tiles[blockIdx.x];
     const uint64_t *__restrict__ keys = t.keys;
+    const uint64_t set0 = bucket_acc[t.bbase];  // the set's first slot
+    if (t.nb <= priv_max && bucket_acc[t.bbase + t.nb] - set0 <= 0xffffffffull) {
+        const uint32_t nb = (uint32_t)t.nb;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_pos[i] = 0;
+        __syncthreads();
+        for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x)
+            atomicAdd(&s_pos[__umul64hi(keys[2 * i], t.nb)], 1u);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+            const uint32_t k = s_pos[i];
+            if (k) s_pos[i] = (uint32_t)(bucket_acc[t.bbase + i] - set0) + atomicAdd(&cursor[t.bbase + i], k);
+        }
+        __syncthreads();
+        uint64_t *__restrict__ out = sorted + set0;
+        for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x)
+            out[atomicAdd(&s_pos[__umul64hi(keys[2 * i], t.nb)], 1u)] = keys[2 * i + 1];
+        return;
+    }
     for (uint64_t i = t.k0 + threadIdx.x; i < t.k1; i += blockDim.x) {
         const uint64_t b = t.bbase + __umul64hi(keys[2 * i], t.nb);
         const uint32_t p = atomicAdd(&cursor[b], 1u);

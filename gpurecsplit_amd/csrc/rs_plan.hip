@@ -280,7 +280,7 @@ static void plan_enqueue(rs_plan *p, const uint64_t *keys, hipStream_t s) {
     scan_level(s, p->counts, true, NB, p->acc, p->scan_tmp);
     scan_level(s, p->node_cnt, false, NB, p->node_base, p->scan_tmp);
     k_plan_census<<<1, 256, 0, s>>>(p->size_hist, p->census, p->list_count, p->list_base, p->err);
-    k_scatter<<<p->n_tiles, 256, 0, s>>>(p->tiles, p->acc, p->cursor, p->sorted);
+    k_scatter<<<p->n_tiles, 256, (size_t)priv_max * 4, s>>>(p->tiles, p->acc, p->cursor, p->sorted, priv_max);
     // buckets up to the likely largest size with a small LDS footprint, larger
     // ones (possible, vanishingly rare) by a second pass with room for 4096 keys
     uint32_t sort_p = 1;
