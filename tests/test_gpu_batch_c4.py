@@ -46,18 +46,19 @@ def test_4096_uneven_sets_l12_b100(env):
     sys.stdout.flush()
 
 
-def test_batch_set_above_8192_buckets_uses_global_histogram(env):
-    """A set of 1,000,000 keys at b = 100 has 10,000 buckets, more than the LDS
-    histogram holds: its bucket counts go through global atomics.  Every set's
-    bytes equal the oracle's build of that set alone."""
+def test_batch_set_above_16384_buckets_uses_global_atomics(env):
+    """A set of 1,700,000 keys at b = 100 has 17,000 buckets, more than the LDS
+    histogram and slot reservation hold: its bucket counts and slots go through
+    global atomics, the small sets' through LDS.  Every set's bytes equal the
+    oracle's build of that set alone."""
     torch, gra, gen = env
-    sizes = [1_000_000, 5000, 300]
+    sizes = [1_700_000, 5000, 300]
     keys = np.concatenate([gen(500 + i, s).cpu().numpy().view(np.uint64) for i, s in enumerate(sizes)])
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     d = torch.from_numpy(keys.view(np.int64)).cuda()
     print(f"sets {sizes}", flush=True)
     rss, st = gra.build_batch(d, off, 100, leaf_size=12)
-    assert st["num_buckets"] == 10000 + 50 + 3
+    assert st["num_buckets"] == 17000 + 50 + 3
     for i, rs in enumerate(rss):
         k = keys[off[i]:off[i + 1]]
         assert rs.serialize() == OracleMPHF.build(k, 12, 100, threads=8).serialize(), f"set {i}"
@@ -81,3 +82,25 @@ def test_batch_more_than_4M_buckets_device_scan(env):
     one = gra.GPURecSplit(d[: sizes[0] + sizes[1]], 1, leaf_size=8)  # plain build, 4.5M buckets
     k = keys[: sizes[0] + sizes[1]]
     assert one.serialize() == OracleMPHF.build(k, 8, 1, threads=8).serialize()
+
+
+def test_batch_of_34M_keys_uses_large_tiles(env):
+    """34.4M keys in one batch (more than two 64K-key tiles per CU, so the
+    histogram and scatter take 64K-key tiles): one set of 2M keys above the LDS
+    bucket limit and 108 sets of 300,000 keys below it.  Every set is a bijection
+    (device queries) and two sets are byte-identical to the oracle's builds."""
+    torch, gra, gen = env
+    sizes = [2_000_000] + [300_000] * 108
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    print(f"{len(sizes)} sets, {n} keys", flush=True)
+    d = gen(77, n)
+    rss, st = gra.build_batch(d, off, 100, leaf_size=12)
+    assert st["num_buckets"] == 20000 + 108 * 3000
+    for i, rs in enumerate(rss):
+        a, b = int(off[i]), int(off[i + 1])
+        idx = rs.query_device(d[a:b])
+        assert torch.equal(torch.sort(idx).values, torch.arange(b - a, device=idx.device)), f"set {i}"
+    for i in (1, 108):
+        k = d[int(off[i]):int(off[i + 1])].cpu().numpy().view(np.uint64)
+        assert rss[i].serialize() == OracleMPHF.build(k, 12, 100, threads=8).serialize(), f"set {i}"
