@@ -498,20 +498,25 @@ static void launch_kind(hipStream_t s, const uint64_t *keys, const SearchArgs &a
                         size_t lds) {
     if constexpr (KIND == kUpper || KIND == kLower) {
         if (a.bt_m && wide && lut && !R64)
-            k_search<KIND, U, true, true, SK, false, false, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
+            k_search<KIND, U, true, true, SK, false, 0, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
         else if (a.bt_m && lut && !R64)
-            k_search<KIND, U, false, true, SK, false, false, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
+            k_search<KIND, U, false, true, SK, false, 0, true><<<blocks, kBtThreads, lds, s>>>(keys, a);
         else if (wide && lut) k_search<KIND, U, true, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else if (wide) k_search<KIND, U, true, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
         else if (lut) k_search<KIND, U, false, true, SK, R64><<<blocks, 256, lds, s>>>(keys, a);
         else k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     } else if constexpr (KIND == kLeafRot) {
-        if (a.bt_m && !R64 && a.leaf <= 16)
-            k_search<KIND, U, false, false, SK, false, true, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
+        const size_t canon_lds = ((size_t)2) << a.leaf;  // least-rotation table, leaves of <= kNecklaceMax keys
+        if (a.bt_m && !R64 && a.leaf <= kNecklaceMax)
+            k_search<KIND, U, false, false, SK, false, 2, true><<<blocks, kBtThreadsLeaf, canon_lds, s>>>(keys, a);
+        else if (a.bt_m && !R64 && a.leaf <= 16)
+            k_search<KIND, U, false, false, SK, false, 1, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
         else if (a.bt_m && !R64)
-            k_search<KIND, U, false, false, SK, false, false, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
+            k_search<KIND, U, false, false, SK, false, 0, true><<<blocks, kBtThreadsLeaf, 0, s>>>(keys, a);
+        else if (a.leaf <= kNecklaceMax)
+            k_search<KIND, U, false, false, SK, R64, 2><<<blocks, 256, canon_lds, s>>>(keys, a);
         else if (a.leaf <= 16)
-            k_search<KIND, U, false, false, SK, R64, true><<<blocks, 256, 0, s>>>(keys, a);
+            k_search<KIND, U, false, false, SK, R64, 1><<<blocks, 256, 0, s>>>(keys, a);
         else
             k_search<KIND, U, false, false, SK, R64><<<blocks, 256, 0, s>>>(keys, a);
     } else {

@@ -672,13 +672,17 @@ __device__ __forceinline__ uint64_t group_aggr_bt(const uint64_t *__restrict__ k
     return kNone;
 }
 
+// Rotation leaves of at most this many keys test candidates through the table of
+// least rotations (2^L two-byte entries of LDS).
+constexpr uint32_t kNecklaceMax = 12;
+
 // Full leaf with rotation fitting (SIMDRecSplit.hpp:452-509): unit j is the base
 // seed j*L; keys with even `second` set bits of the left mask, odd ones of the
 // right mask; the lowest r with left | rotl(right, r) == 2^L-1 gives seed j*L+r.
 // The partition places a full leaf's even keys first: kp[0, nl) even, kp[nl, L) odd.
-template <int U, bool SK, bool R64, bool RB, bool BT = false>
+template <int U, bool SK, bool R64, int RB, bool BT = false>
 __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ kp, uint32_t L, uint32_t nl,
-                                                   uint32_t nr, uint64_t u0) {
+                                                   uint32_t nr, uint64_t u0, const uint16_t *canon) {
     // BT: position (h16 * L) >> 16 of every 16-bit hash read from the byte table
     const uint8_t *bt = bt_table<BT>();
     const uint64_t S = c_start_seed[kSeedLeaf];
@@ -744,6 +748,30 @@ __device__ __forceinline__ uint64_t group_leaf_rot(const uint64_t *__restrict__ 
             each_hash<U, SK>(kp, 0, nl, sd, [&](int u, uint32_t h) { mk[u] |= 1u << (__umul24(h, L) >> 16); });
             each_hash<U, SK>(kp, nl, L, sd,
                              [&](int u, uint32_t h) { mk[u] |= 1u << ((__umul24(h, L) + (16u << 16)) >> 16); });
+        }
+        if constexpr (RB == 2) {
+            // Leaves of at most kNecklaceMax keys: canon[x] is the least rotation of
+            // the L-bit mask x.  Some rotation of the right mask equals the left
+            // mask's complement exactly when both have the same least rotation
+            // (which also makes both halves collision-free: a collision changes a
+            // popcount), so one pair of table reads per candidate replaces the list
+            // and its steps; the first matching candidate in unit order then finds
+            // its lowest fitting rotation across the wave (lane r tests rotation r).
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool match = canon[mk[u] >> 16] == canon[~mk[u] & full];
+                const unsigned long long bal = __ballot(match);
+                if (bal) {
+                    const int l = __ffsll((long long)bal) - 1;
+                    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)mk[u], l);
+                    const uint32_t bl = e & 0xFFFFu, br = e >> 16;
+                    const uint32_t r = lane < L ? lane : 0;
+                    const uint32_t rot = r == 0 ? br : (((br << r) | (br >> (L - r))) & full);
+                    const unsigned long long fit = __ballot(lane < L && (bl | rot) == full);
+                    return (u0 + (uint64_t)(u * 64 + l)) * L + (uint64_t)(__ffsll((long long)fit) - 1);
+                }
+            }
+            return kNone;
         }
         // one list per wave of the workgroup: masks, and unit indexes (< 256) as bytes
         constexpr int kWaves = (BT ? kBtThreadsLeaf : 256) / 64;
@@ -828,12 +856,13 @@ __device__ __forceinline__ bool has_work(const NodeState *st, uint64_t units_per
     return nc * units_per_chunk * scale < res;
 }
 
-// RB (rotation leaves of at most 16 keys): packed masks and the candidate test
-// batched through LDS, with registers held to five waves per SIMD.  BT
+// RB (rotation leaves of at most 16 keys): packed masks, and the candidate test
+// batched through LDS (RB = 1) or, for at most kNecklaceMax keys, through the
+// table of least rotations (RB = 2), with registers held to five waves per SIMD.  BT
 // (aggregation levels, 32-bit counters, 16-bit remap): the byte table for nodes of
 // bt_m keys, workgroups of kBtThreads threads, two per CU (registers held to the
 // kBtThreads / 128 waves per SIMD that two such workgroups need).
-template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, bool RB = false, bool BT = false>
+template <int KIND, int U, bool WIDE, bool LUT, bool SK, bool R64, int RB = 0, bool BT = false>
 __global__ __launch_bounds__(BT ? bt_threads(KIND) : 256, BT ? bt_threads(KIND) / 128 : (RB ? 5 : 1)) void k_search(const uint64_t *__restrict__ keys,
                                                                                   SearchArgs a) {
     static_assert(!BT || ((((KIND == kUpper || KIND == kLower) && LUT && !RB) || KIND == kLeafRot) && !R64),
@@ -885,6 +914,20 @@ __global__ __launch_bounds__(BT ? bt_threads(KIND) : 256, BT ? bt_threads(KIND) 
 #pragma unroll
             for (uint32_t j = 0; j < 4; j++) v |= (((w * 4 + j) * L) >> 16) << (8 * j);
             bt_w[w] = v;
+        }
+        __syncthreads();
+    }
+    if constexpr (KIND == kLeafRot && RB == 2) {
+        // least rotation of every L-bit mask, in the dynamic LDS (2^L entries)
+        uint16_t *canon = (uint16_t *)s_lut;
+        const uint32_t L = a.leaf, full = (1u << L) - 1;
+        for (uint32_t x = threadIdx.x; x <= full; x += blockDim.x) {
+            uint32_t c = x, y = x;
+            for (uint32_t t = 1; t < L; t++) {
+                y = ((y << 1) | (y >> (L - 1))) & full;
+                c = y < c ? y : c;
+            }
+            canon[x] = (uint16_t)c;
         }
         __syncthreads();
     }
@@ -1085,7 +1128,8 @@ __global__ __launch_bounds__(BT ? bt_threads(KIND) : 256, BT ? bt_threads(KIND) 
             else if (KIND == kUpper || KIND == kLower)
                 hit = group_aggr<U, WIDE, LUT, SK, R64>(kp, m, sconst, a.field_bits, a.magic, T, cmask, lut_hi, two,
                                                         u0);
-            else if (KIND == kLeafRot) hit = group_leaf_rot<U, SK, R64, RB, BT>(kp, m, aux0, aux1, u0);
+            else if (KIND == kLeafRot)
+                hit = group_leaf_rot<U, SK, R64, RB, BT>(kp, m, aux0, aux1, u0, (const uint16_t *)s_lut);
             else hit = group_leaf_brute<U, SK, R64>(kp, m, aux0, u0);
             if (hit != kNone) break;
         }
